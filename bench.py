@@ -1,0 +1,157 @@
+#!/usr/bin/env python3
+"""Throughput of the fused 2D IB-LBM vortex-induced-vibration step on one MI355X.
+
+Workload (the headline configuration): D2Q9, 1024 x 1024 cells, MRT collision
+with Guo forcing, a D = 40 cylinder at (256, 512) carried by 512 markers,
+Peskin 4-point kernel, 3 multi-direct-forcing iterations, FP64 body-force
+reduction and Newmark update, equilibrium inlet, pressure-NEE outlet,
+periodic top / bottom, FP32 populations.  A "step" is one full IB-LBM time
+step; ``value`` is GLUPS = cells * steps / seconds / 1e9 with the state
+resident in HBM.
+
+Also reported on the same line:
+  roofline      : the bulk stream-collide kernel's achieved algorithmic bandwidth
+                  (72 B per cell in FP32 / 36 B in 16-bit storage) from HIP events
+                  recorded around every launch in the timed region, vs 8 TB/s
+  variants      : FP16 / BF16 shifted-storage GLUPS of the same step
+  cpu_baseline  : the oracle (NumPy FP32 restatement of the reference step),
+                  single-threaded on a bounded sample of the same workload
+"""
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+
+
+def kernel_times(lib, n):
+    ms = (ctypes.c_float * n)()
+    got = ctypes.c_int32(0)
+    rc = lib.vsx_timer_stop(ms, n, ctypes.byref(got))
+    assert rc == 0, lib.vsx_last_error()
+    return np.array(ms[: got.value], dtype=np.float64)
+
+
+def run_gpu(args, storage, steps, warmup):
+    import torch
+    from vivsim_amd import _lib, lbm, sim
+
+    case = sim.VIVCase(nx=args.nx, ny=args.ny)
+    solver = sim.VIV2D(case, storage=storage)
+    solver.set_body(v=(0.0, 1e-3 * case.U0))
+    u = torch.as_tensor(case.initial_u(), dtype=torch.float32, device="cuda")
+    f = lbm.get_equilibrium(torch.ones(case.nx, case.ny, device="cuda"), u)
+    f, _, _ = solver.run(f, warmup, history=False)
+    torch.cuda.synchronize()
+    lib = _lib.lib()
+    assert lib.vsx_timer_start(steps) == 0
+    t0 = time.perf_counter()
+    out, hd, hh = solver.run(f, steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    kt = kernel_times(lib, steps)
+    assert torch.isfinite(out).all().item(), "non-finite state"
+    cells = case.nx * case.ny
+    sec = t1 - t0
+    bulk_cells = (case.nx - 2) * case.ny            # left / right wall columns run in the edge kernel
+    bpc = 72 if storage == "f32" else 36
+    mean_ms = float(kt.mean())
+    return dict(glups=cells * steps / sec / 1e9, ms_per_step=sec / steps * 1e3, kernel_ms=mean_ms,
+                kernel_launches=int(kt.size), bytes_per_launch=bpc * bulk_cells,
+                achieved_gbs=bpc * bulk_cells / (mean_ms * 1e-3) / 1e9,
+                lift_last=float(hh[-1, 1].item()), d_last=float(hd[-1, 1].item()))
+
+
+def run_cpu(args, seconds_budget):
+    """Oracle VIV step (float32, one thread) on the same 1024^2 workload."""
+    from threadpoolctl import threadpool_limits
+
+    from oracle import steps as S
+
+    case = S.VIV2D(nx=args.nx, ny=args.ny)
+    f = case.initial_f(np.float32)
+    d, v, a = np.zeros(2), np.array([0.0, 1e-3 * case.U0]), np.zeros(2)
+    n = 0
+    with threadpool_limits(1):
+        t0 = time.perf_counter()
+        while True:
+            f, d, v, a, h = S.viv_step(case, f, d, v, a)
+            n += 1
+            if time.perf_counter() - t0 > seconds_budget or n >= args.cpu_max_steps:
+                break
+        sec = time.perf_counter() - t0
+    return dict(value=case.nx * case.ny * n / sec / 1e9, unit="GLUPS", cores=1, kind="port",
+                sample=f"{n} oracle VIV steps (NumPy float32, 1 thread) on the {args.nx}x{args.ny} workload, "
+                       f"{sec:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--nx", type=int, default=1024)
+    ap.add_argument("--ny", type=int, default=1024)
+    ap.add_argument("--storage", default="f32", choices=["f32", "f16s", "bf16s"])
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-max-steps", type=int, default=60)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-variants", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: this build runs the IB-LBM step on a single MI355X "
+              "(multi-GPU slab decomposition is design-only); rerun with --gpus 1", file=sys.stderr)
+        return 0
+
+    import torch
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+
+    main_run = run_gpu(args, args.storage, args.steps, args.warmup)
+    variants = {}
+    if not args.no_variants:
+        for st in ("f32", "f16s", "bf16s"):
+            if st != args.storage:
+                r = run_gpu(args, st, args.steps, args.warmup)
+                variants[st] = dict(glups=round(r["glups"], 3), ms_per_step=round(r["ms_per_step"], 5),
+                                    kernel_us=round(r["kernel_ms"] * 1e3, 3), achieved_gbs=round(r["achieved_gbs"], 1))
+    cpu = None if args.no_cpu else run_cpu(args, args.cpu_seconds)
+
+    line = {
+        "metric": "GLUPS & %HBM peak, 2D IB-LBM 1024² FP32 vs FP16-store; adjoint/fwd time ratio",
+        "value": round(main_run["glups"], 3),
+        "unit": "GLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(main_run["ms_per_step"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.storage == "f32" else f"{args.storage} storage, f32 arithmetic",
+        "data": "synthetic (seeded +-1% velocity perturbation on a uniform inflow)",
+        "config": {"workload": f"viv2d_d2q9_mrt_guo_{args.nx}x{args.ny}_D40_m512_mdf3",
+                   "nx": args.nx, "ny": args.ny, "markers": 512, "mdf_iterations": 3, "collision": "mrt",
+                   "forcing": "guo", "kernel": "peskin_4pt", "storage": args.storage, "structure": "fp64 newmark"},
+        "roofline": {"bound": "hbm", "achieved": round(main_run["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(main_run["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_bulk (pull-stream + MRT + Guo, FULL phase)",
+                     "kernel_us": round(main_run["kernel_ms"] * 1e3, 3),
+                     "bytes_per_launch": main_run["bytes_per_launch"], "launches": main_run["kernel_launches"]},
+        "variants": variants,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

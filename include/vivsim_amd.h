@@ -1,0 +1,258 @@
+/*
+ * vivsim_amd — C ABI of the MI355X (gfx950) IB-LBM kernels.
+ *
+ * The reference (haimingz/vivsim) is a pure-Python/JAX library whose public
+ * surface is the module functions of vivsim.lbm, vivsim.lbm3d, vivsim.ib,
+ * vivsim.ib3d and vivsim.dyn.  Each entry point below replaces one of those
+ * functions (cited as reference file:line) or one of the user-composed step
+ * functions of the reference examples.  The Python modules in
+ * vivsim_amd/{lbm,lbm3d,ib,ib3d,dyn} bind these symbols through ctypes.
+ *
+ * Conventions
+ *  - All array arguments are DEVICE pointers to contiguous C-order buffers in
+ *    the reference layout: f[Q][NX][NY] (D2Q9) / f[Q][NX][NY][NZ] (D3Q19),
+ *    rho[...], u[D][...], marker arrays [M][C], stencils [M][S].
+ *  - Field dtype float32, stencil indices int32, structural state float64.
+ *  - `stream` is a hipStream_t (NULL = default stream).  No entry point
+ *    synchronises the host or allocates device memory; buffers are owned by
+ *    the caller.
+ *  - Every function returns a status code (VSX_OK on success).  The message
+ *    of the last failure on the calling thread is vsx_last_error().
+ *    VSX_ERR_VALUE maps to Python ValueError, VSX_ERR_KEY to KeyError (the
+ *    reference raises KeyError for an unknown wall `loc`, e.g.
+ *    lbm/boundary/nee.py:43), VSX_ERR_HIP to RuntimeError.
+ */
+#ifndef VIVSIM_AMD_H
+#define VIVSIM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ enums */
+enum { VSX_OK = 0, VSX_ERR_VALUE = 1, VSX_ERR_KEY = 2, VSX_ERR_HIP = 3 };
+
+/* wall locations: lbm/lattice.py:64-109, lbm3d/lattice.py:64-125 */
+enum { VSX_LOC_LEFT = 0, VSX_LOC_RIGHT = 1, VSX_LOC_BOTTOM = 2, VSX_LOC_TOP = 3,
+       VSX_LOC_BACK = 4, VSX_LOC_FRONT = 5 };
+
+/* wall schemes: boundary/{nee,nebb,eq,bb}.py */
+enum { VSX_WALL_NEE = 0, VSX_WALL_NEBB = 1, VSX_WALL_EQ = 2, VSX_WALL_BB = 3,
+       VSX_WALL_SPECULAR = 4 };
+
+/* wall wrappers: boundary/_helpers.py:180-239 (plain core, velocity, pressure, force-corrected) */
+enum { VSX_MODE_PLAIN = 0, VSX_MODE_VELOCITY = 1, VSX_MODE_PRESSURE = 2,
+       VSX_MODE_FORCE_CORRECTED = 3 };
+
+enum { VSX_COLL_BGK = 0, VSX_COLL_MRT = 1, VSX_COLL_KBC = 2, VSX_COLL_REG = 3 };
+enum { VSX_FORCE_NONE = 0, VSX_FORCE_GUO = 1, VSX_FORCE_EDM = 2 };
+
+/* ib/kernels.py:4-61 (+ a 2-point hat kernel, not in the reference) */
+enum { VSX_KERNEL_PESKIN_2PT = 0, VSX_KERNEL_PESKIN_3PT = 1, VSX_KERNEL_PESKIN_4PT = 2,
+       VSX_KERNEL_COSINE_4PT = 3 };
+
+/* population storage of the fused steps: FP32, or FP16 / BF16 deviations f - w_i */
+enum { VSX_STORE_F32 = 0, VSX_STORE_F16S = 1, VSX_STORE_BF16S = 2 };
+
+/* phases of the fused pull-scheme step (the stored state is post-collision) */
+enum { VSX_PHASE_COLLIDE = 0,   /* read f locally (post-stream state), collide, write          */
+       VSX_PHASE_FULL = 1,      /* pull-stream + walls, collide, write                          */
+       VSX_PHASE_STREAM = 2 };  /* pull-stream + walls, write (recovers the reference's state)  */
+
+/* ------------------------------------------------------------------ structs */
+
+/* One domain wall.  Scalars apply when the matching *_arr pointer is NULL.
+ * Per-node arrays have the wall's shape: (N,) in 2D, (N1, N2) in 3D with the
+ * two remaining axes in x, y, z order (boundary/_helpers.py:53-77). */
+typedef struct VsxWall {
+  int32_t kind, mode, loc, reserved0;
+  float rho, ux, uy, uz;
+  float gx, gy, gz, reserved1;
+  const float *rho_arr, *ux_arr, *uy_arr, *uz_arr;
+  const float *gx_arr, *gy_arr, *gz_arr;
+} VsxWall;
+
+/* Parameters of a fused LBM step (2D: nz = 1). */
+typedef struct VsxLbm {
+  int32_t nx, ny, nz;
+  int32_t collision, forcing, storage;
+  float omega;
+  float mrt_rates[19];     /* diag(S) in the reference's moment basis (collision/mrt.py:28-44) */
+  int32_t n_walls;         /* walls[0..n_walls) are applied in order after streaming           */
+  VsxWall walls[6];
+  const float *g;          /* optional full body-force field [D][NX][NY(][NZ)] (device)        */
+  float g_uniform[3];      /* uniform body force added to g                                     */
+  int32_t reserved;
+} VsxLbm;
+
+/* 2D immersed body (rigid cylinder on springs) coupled to a VsxLbm step.
+ * Step composition: examples/2d/vortex_induced_vibration.py:96-148. */
+typedef struct VsxIB2D {
+  int32_t n_markers, kernel, n_iter, win;   /* win: side of the square IB window            */
+  const double *marker_x, *marker_y;        /* initial marker coordinates (device, FP64)    */
+  const float *ds_arr;                      /* per-marker ds (device) or NULL -> ds          */
+  float ds;
+  int32_t dof_mask;                         /* bit0: x DOF moves, bit1: y DOF moves          */
+  double win_x0, win_y0;                    /* window origin before displacement            */
+  double mass, stiffness, damping, area;    /* Newmark (dyn.py:5-51), h += a * area          */
+  double dt, gamma, beta;
+  double *body;                             /* device FP64 [a0 a1 v0 v1 d0 d1], in/out       */
+  float *g_win;                             /* device scratch [2][win][win]                  */
+  double *hist_d, *hist_h;                  /* optional device histories [n_steps][2]        */
+  int32_t *win_origin;                      /* device scratch [2]                            */
+} VsxIB2D;
+
+/* ------------------------------------------------------------------ library */
+const char *vsx_version(void);
+const char *vsx_last_error(void);
+int vsx_device_sync(void);
+/* Instrumentation (no counterpart in the reference): between start and stop the
+ * fused drivers record a HIP event pair, on their stream, around every
+ * VSX_PHASE_FULL bulk kernel (at most max_pairs); stop synchronises on the last
+ * pair and returns the per-launch durations in milliseconds. */
+int vsx_timer_start(int32_t max_pairs);
+int vsx_timer_stop(float *ms, int32_t max, int32_t *n_out);
+
+/* ------------------------------------------------------------------ D2Q9 / D3Q19 pure functions
+ * n = number of cells (product of the spatial shape); arrays are [Q][n], [D][n], [n].
+ * Suffix d2q9 <-> vivsim/lbm, d3q19 <-> vivsim/lbm3d. */
+
+/* lbm/basic.py:60-85, lbm3d/basic.py:52-87 */
+int vsx_d2q9_streaming(const float *f, float *out, int32_t nx, int32_t ny, void *stream);
+int vsx_d3q19_streaming(const float *f, float *out, int32_t nx, int32_t ny, int32_t nz, void *stream);
+/* lbm/basic.py:88-111, lbm3d/basic.py:90-106 */
+int vsx_d2q9_get_macroscopic(const float *f, float *rho, float *u, int64_t n, void *stream);
+int vsx_d3q19_get_macroscopic(const float *f, float *rho, float *u, int64_t n, void *stream);
+/* lbm/basic.py:175-194, lbm3d/basic.py:155-167 : out = g * 0.5 / rho ; g [comps][n],
+ * rho [n] or NULL -> rho_scalar */
+int vsx_velocity_correction(const float *g, const float *rho, float rho_scalar, float *out, int32_t comps,
+                            int64_t n, void *stream);
+/* lbm/basic.py:114-135, lbm3d/basic.py:109-130 */
+int vsx_d2q9_get_equilibrium(const float *rho, const float *u, float *feq, int64_t n, void *stream);
+int vsx_d3q19_get_equilibrium(const float *rho, const float *u, float *feq, int64_t n, void *stream);
+/* lbm/basic.py:138-156, lbm3d/basic.py:133-146 */
+int vsx_d2q9_collision_bgk(const float *f, const float *feq, float omega, float *out, int64_t n, void *stream);
+int vsx_d3q19_collision_bgk(const float *f, const float *feq, float omega, float *out, int64_t n, void *stream);
+/* lbm/collision/mrt.py:66-88, lbm3d/collision/mrt.py:83-98 ; op = host float[Q*Q] */
+int vsx_d2q9_collision_mrt(const float *f, const float *feq, const float *op, float *out, int64_t n, void *stream);
+int vsx_d3q19_collision_mrt(const float *f, const float *feq, const float *op, float *out, int64_t n, void *stream);
+/* lbm/collision/kbc.py:10-59, lbm3d/collision/kbc.py:8-42 */
+int vsx_d2q9_collision_kbc(const float *f, const float *feq, float omega, float *out, int64_t n, void *stream);
+int vsx_d3q19_collision_kbc(const float *f, const float *feq, float omega, float *out, int64_t n, void *stream);
+/* lbm/collision/reg.py:4-49, lbm3d/collision/reg.py:45-62 */
+int vsx_d2q9_collision_reg(const float *f, const float *feq, float omega, float *out, int64_t n, void *stream);
+int vsx_d3q19_collision_reg(const float *f, const float *feq, float omega, float *out, int64_t n, void *stream);
+/* lbm/forcing/guo.py:6-33, lbm3d/forcing/guo.py:13-38 */
+int vsx_d2q9_get_guo_forcing_term(const float *g, const float *u, float *out, int64_t n, void *stream);
+int vsx_d3q19_get_guo_forcing_term(const float *g, const float *u, float *out, int64_t n, void *stream);
+/* lbm/forcing/guo.py:38-57, lbm3d/forcing/guo.py:41-56 */
+int vsx_d2q9_forcing_guo_bgk(const float *f, const float *g, const float *u, float omega, float *out, int64_t n, void *stream);
+int vsx_d3q19_forcing_guo_bgk(const float *f, const float *g, const float *u, float omega, float *out, int64_t n, void *stream);
+/* lbm/forcing/guo.py:82-107, lbm3d/forcing/guo.py:71-87 ; op = host float[Q*Q] */
+int vsx_d2q9_forcing_guo_mrt(const float *f, const float *g, const float *u, const float *op, float *out, int64_t n, void *stream);
+int vsx_d3q19_forcing_guo_mrt(const float *f, const float *g, const float *u, const float *op, float *out, int64_t n, void *stream);
+/* lbm/forcing/edm.py:4-31, lbm3d/forcing/edm.py:6-26 */
+int vsx_d2q9_forcing_edm(const float *f, const float *g, const float *u, float *out, int64_t n, void *stream);
+int vsx_d3q19_forcing_edm(const float *f, const float *g, const float *u, float *out, int64_t n, void *stream);
+/* lbm/boundary/bb.py:98-110, lbm3d/boundary/bb.py:56-59 ; mask = uint8 [n] */
+int vsx_d2q9_obstacle_bounce_back(const float *f, const uint8_t *mask, float *out, int64_t n, void *stream);
+int vsx_d3q19_obstacle_bounce_back(const float *f, const uint8_t *mask, float *out, int64_t n, void *stream);
+
+/* One wall of any kind/mode on a full field (out = f with the wall line rewritten;
+ * out may alias f).  f_pre is the pre-streaming field (bounce-back / specular only).
+ *   boundary_{nee,nebb,equilibrium}        lbm/boundary/{nee,nebb,eq}.py (+ 3D)
+ *   boundary_{velocity,pressure,force_corrected}_*   boundary/_helpers.py:180-239
+ *   boundary_bounce_back / specular_reflection        lbm/boundary/bb.py:15-95, lbm3d/boundary/bb.py:9-53 */
+int vsx_d2q9_boundary(const VsxWall *wall, const float *f_pre, const float *f, float *out,
+                      int32_t nx, int32_t ny, void *stream);
+int vsx_d3q19_boundary(const VsxWall *wall, const float *f_pre, const float *f, float *out,
+                       int32_t nx, int32_t ny, int32_t nz, void *stream);
+
+/* lbm/boundary/cbc.py:14-53, lbm3d/boundary/cbc.py:16-51 : rho_next [N], u_next [D][N] */
+int vsx_d2q9_boundary_characteristic(const float *rho, const float *u, int32_t loc, float *rho_next,
+                                     float *u_next, int32_t nx, int32_t ny, void *stream);
+int vsx_d3q19_boundary_characteristic(const float *rho, const float *u, int32_t loc, float *rho_next,
+                                      float *u_next, int32_t nx, int32_t ny, int32_t nz, void *stream);
+
+/* ------------------------------------------------------------------ fused steps
+ * A user step "macroscopic -> [IB] -> collision -> forcing -> streaming -> walls"
+ * (README.md:105-121; examples/2d/lid_driven_cavity.py:55-64;
+ * examples/2d/poiseuille_channel.py:80-128) as one pull-scheme pass.
+ * src / dst use the storage of p->storage, except that VSX_PHASE_COLLIDE reads
+ * float32 when src_f32 != 0 and VSX_PHASE_STREAM writes float32 when dst_f32 != 0. */
+int vsx_d2q9_step(const VsxLbm *p, const void *src, int32_t src_f32, void *dst, int32_t dst_f32,
+                  int32_t phase, void *stream);
+int vsx_d3q19_step(const VsxLbm *p, const void *src, int32_t src_f32, void *dst, int32_t dst_f32,
+                   int32_t phase, void *stream);
+
+/* n_steps reference steps (the jitted lax.scan chunk driver, e.g.
+ * examples/2d/lid_driven_cavity.py:67-73): f_out = step^n(f_in).
+ * work_a / work_b hold NX*NY(*NZ)*Q populations in p->storage; f_out may alias f_in. */
+int vsx_d2q9_run(const VsxLbm *p, const float *f_in, float *f_out, void *work_a, void *work_b,
+                 int32_t n_steps, void *stream);
+int vsx_d3q19_run(const VsxLbm *p, const float *f_in, float *f_out, void *work_a, void *work_b,
+                  int32_t n_steps, void *stream);
+
+/* n_steps of the 2D IB-LBM VIV step (examples/2d/vortex_induced_vibration.py:96-157):
+ * per step: macroscopic -> IB window -> stencil -> multi-direct forcing -> body
+ * force (FP64) -> Newmark (FP64) -> collision + forcing -> streaming -> walls. */
+int vsx_viv2d_run(const VsxLbm *p, const VsxIB2D *ib, const float *f_in, float *f_out,
+                  void *work_a, void *work_b, int32_t n_steps, void *stream);
+
+/* ------------------------------------------------------------------ immersed boundary */
+
+/* ib/kernels.py:4-61 : out[i] = phi(r[i]) */
+int vsx_ib_kernel(int32_t kernel, const float *r, float *out, int64_t n, void *stream);
+/* ib/stencil.py:7-51 : W, I [m][(2 radius)^2] */
+int vsx_ib_get_ib_stencil2d(const float *marker_x, const float *marker_y, int32_t m, int32_t ny,
+                            int32_t kernel, int32_t radius, float *W, int32_t *I, void *stream);
+/* ib3d/stencil.py:7-59 : coords [m][3]; W, I [m][(2 radius)^3] */
+int vsx_ib_get_ib_stencil3d(const float *coords, int32_t m, int32_t nx, int32_t ny, int32_t nz,
+                            int32_t kernel, int32_t radius, float *W, int32_t *I, void *stream);
+/* ib/stencil.py:54-78 : out[m][c] = sum_s W[m][s] grid[c][I[m][s]]  (grid [c][n_grid]) */
+int vsx_ib_interpolate(const float *grid, int32_t c, int64_t n_grid, const float *W, const int32_t *I,
+                       int32_t m, int32_t s, float *out, void *stream);
+/* ib/stencil.py:81-110 : out = grid_in + scatter(marker[m][c] W[m][s] -> I[m][s]); out may alias grid_in */
+int vsx_ib_spread(const float *marker, const float *grid_in, float *out, int32_t c, int64_t n_grid,
+                  const float *W, const int32_t *I, int32_t m, int32_t s, void *stream);
+/* ib/mdf.py:10-64 : grid_force [c][n_grid], marker_force [m][c]; ds_arr NULL -> scalar ds.
+ * work: device scratch of 3*m*c floats. */
+int vsx_ib_multi_direct_forcing(const float *grid_u, int32_t c, int64_t n_grid, const float *W,
+                                const int32_t *I, int32_t m, int32_t s, const float *u_target,
+                                const float *ds_arr, float ds, int32_t n_iter, float *grid_force,
+                                float *marker_force, float *work, void *stream);
+/* ib/geometry.py:6-43 : coords [m][2] */
+int vsx_ib_get_area(const float *coords, int32_t m, float *out, void *stream);
+int vsx_ib_get_ds(const float *coords, int32_t m, int32_t closed, float *out, void *stream);
+/* ib3d/geometry.py:6-67 : verts [nv][3], faces [nf][3] int32 (vertex indices: gathers clamp,
+ * the ds scatter-add drops out-of-range vertices, as JAX indexing does) */
+int vsx_ib3d_get_triangle_areas(const float *verts, int32_t nv, const int32_t *faces, int32_t nf, float *out,
+                                void *stream);
+int vsx_ib3d_get_surface_area(const float *verts, int32_t nv, const int32_t *faces, int32_t nf, float *out,
+                              void *stream);
+int vsx_ib3d_get_volume(const float *verts, int32_t nv, const int32_t *faces, int32_t nf, float *out, void *stream);
+int vsx_ib3d_get_ds(const float *verts, int32_t nv, const int32_t *faces, int32_t nf, float *out, void *stream);
+
+/* ------------------------------------------------------------------ structure (dyn.py), FP64 */
+/* dyn.py:5-51 : a, v, d, h [ndof]; m, k, c scalars (matrix == 0) or [ndof][ndof] (matrix != 0) */
+int vsx_dyn_newmark(const double *a, const double *v, const double *d, const double *h, int32_t ndof,
+                    const double *m, const double *k, const double *c, int32_t matrix, double dt,
+                    double gamma, double beta, double *a_out, double *v_out, double *d_out, void *stream);
+/* dyn.py:126-136 : out[c] = sum_m h[m][c] (FP64 accumulation of float32 forces) */
+int vsx_dyn_get_force_to_obj(const float *h, int32_t m, int32_t c, double *out, void *stream);
+/* dyn.py:139-154 */
+int vsx_dyn_get_torque_to_obj(const float *x, const float *y, double xc, double yc, const double *d,
+                              const float *h, int32_t m, double *out, void *stream);
+/* dyn.py:68-100 : 2-DOF translation (rotate == 0) or 3-DOF rigid motion (rotate != 0) */
+int vsx_dyn_markers_coords(const float *x0, const float *y0, int32_t m, double xc, double yc,
+                           const double *d, int32_t rotate, float *x, float *y, void *stream);
+/* dyn.py:103-120 : v_markers [m][2] */
+int vsx_dyn_markers_velocity_3dof(const float *x, const float *y, int32_t m, double xc, double yc,
+                                  const double *d, const double *v, float *out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VIVSIM_AMD_H */

@@ -1,0 +1,178 @@
+"""GPU parity of the fused D2Q9 step (vivsim_amd.sim) against the oracle's
+composition of the reference functions (oracle/steps.py).
+
+The fused kernel evaluates the MRT operator in moment space and the walls in
+a pull scheme, so results agree with the float64 composition to float32
+rounding: tolerances below are relative L-inf errors of the population field."""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import lbm2d as O
+from oracle import steps as S
+from tests.gpu_util import assert_close, dev, rel_err, say, to_np
+
+pytestmark = pytest.mark.gpu
+rng = np.random.default_rng(77)
+
+
+@pytest.fixture(scope="module")
+def sim(gpu_lib):
+    from vivsim_amd import sim
+    return sim
+
+
+def rand_state(nx, ny, u0=0.04):
+    rho = 1 + 0.02 * rng.standard_normal((nx, ny))
+    u = u0 * rng.standard_normal((2, nx, ny))
+    return (O.get_equilibrium(rho, u) + 1e-3 * rng.standard_normal((9, nx, ny))).astype(np.float32)
+
+
+WALLSETS = {
+    "none": [],
+    "cavity_nee": S.cavity_walls(0.08),
+    "cavity_nee_lid_first": S.cavity_walls(0.08, order=("top", "left", "right", "bottom")),
+    "viv_like": [dict(kind="eq", mode="plain", loc="left", ux_wall=0.05),
+                 dict(kind="nee", mode="pressure", loc="right", rho_wall=1.0)],
+    "channel": [dict(kind="nebb", mode="velocity", loc="left", ux_wall=0.04),
+                dict(kind="nebb", mode="pressure", loc="right", rho_wall=1.0),
+                dict(kind="bb", loc="bottom"),
+                dict(kind="specular", loc="top", ux_wall=0.01)],
+    "mixed": [dict(kind="nebb", mode="force_corrected", loc="top", ux_wall=0.03, gx_wall=1e-4),
+              dict(kind="eq", mode="velocity", loc="left", ux_wall=0.02),
+              dict(kind="nee", mode="velocity", loc="bottom", ux_wall=0.01),
+              dict(kind="eq", mode="pressure", loc="right", rho_wall=1.0)],
+}
+
+
+def run_oracle(f, n, omega, walls, collision, forcing, g):
+    f = f.astype(np.float64)
+    op = O.get_mrt_collision_operator(omega) if collision == "mrt" else None
+    fop = O.get_mrt_forcing_operator(omega) if collision == "mrt" else None
+    for _ in range(n):
+        f = S.lbm_step2d(f, omega, walls, collision, forcing, g, op, fop)
+    return f
+
+
+@pytest.mark.parametrize("walls", list(WALLSETS))
+@pytest.mark.parametrize("collision", ["bgk", "mrt", "kbc", "reg"])
+def test_walls_and_collisions(sim, walls, collision):
+    nx, ny, n, om = 26, 21, 12, 1.75
+    f0 = rand_state(nx, ny)
+    say(f"fused d2q9 {collision} walls={walls}")
+    got = sim.LBM((nx, ny), om, collision=collision, walls=WALLSETS[walls]).run(dev(f0), n)
+    ref = run_oracle(f0, n, om, WALLSETS[walls], collision, None, None)
+    assert_close(got, ref, 2e-5, f"{collision}/{walls}")
+
+
+@pytest.mark.parametrize("forcing", ["guo", "edm"])
+@pytest.mark.parametrize("collision", ["bgk", "mrt", "kbc", "reg"])
+def test_forcing(sim, collision, forcing):
+    nx, ny, n, om = 30, 24, 10, 1.6
+    f0 = rand_state(nx, ny)
+    g = np.zeros((2, nx, ny), dtype=np.float32)
+    g[:, 8:20, 6:18] = 2e-4 * rng.standard_normal((2, 12, 12))     # localised force (IB-like window)
+    g += np.float32(1e-5)                                            # plus a uniform part
+    walls = WALLSETS["viv_like"]
+    say(f"fused d2q9 {collision}+{forcing}")
+    got = sim.LBM((nx, ny), om, collision=collision, walls=walls, forcing=forcing, g=dev(g)).run(dev(f0), n)
+    ref = run_oracle(f0, n, om, walls, collision, forcing, g.astype(np.float64))
+    assert_close(got, ref, 2e-5, f"{collision}+{forcing}")
+
+
+def test_uniform_force_equals_field(sim):
+    nx, ny = 20, 18
+    f0 = rand_state(nx, ny)
+    walls = WALLSETS["channel"]
+    a = sim.LBM((nx, ny), 1.5, collision="bgk", walls=walls, forcing="guo", g_uniform=(2e-5, 0.0)).run(dev(f0), 6)
+    g = np.zeros((2, nx, ny), dtype=np.float32)
+    g[0] = 2e-5
+    b = sim.LBM((nx, ny), 1.5, collision="bgk", walls=walls, forcing="guo", g=dev(g)).run(dev(f0), 6)
+    assert torch.equal(a, b)
+
+
+def test_step_phases_match_run(sim):
+    """collide -> full^(n-1) -> stream through the low-level entry equals run()."""
+    nx, ny = 24, 20
+    f0 = dev(rand_state(nx, ny))
+    m = sim.LBM((nx, ny), 1.8, collision="mrt", walls=WALLSETS["cavity_nee"])
+    a = m.run(f0, 4)
+    w0, w1 = torch.empty_like(f0), torch.empty_like(f0)
+    m.step(f0, w0, "collide", src_f32=True)
+    m.step(w0, w1, "full")
+    m.step(w1, w0, "full")
+    m.step(w0, w1, "full")
+    out = torch.empty_like(f0)
+    m.step(w1, out, "stream", dst_f32=True)
+    assert torch.equal(a, out)
+    assert torch.equal(m.run(f0, 0), f0)
+
+
+def test_cavity_1000_steps(sim):
+    """Lid-driven cavity 256^2, BGK, NEE x4, U0 = 0.1, nu = 0.01, 1000 steps:
+    float32 fused run vs float64 composition, relative L-inf <= 1e-5."""
+    n, steps, U0 = 256, 1000, 0.1
+    om = O.get_omega(0.01)
+    walls = S.cavity_walls(U0)
+    f0 = O.get_equilibrium(np.ones((n, n)), np.zeros((2, n, n)))
+    say("cavity 256^2 x 1000 (oracle ~20 s)")
+    got = sim.LBM((n, n), om, collision="bgk", walls=walls).run(dev(f0), steps)
+    ref = run_oracle(f0.astype(np.float32), steps, om, walls, "bgk", None, None)
+    e = assert_close(got, ref, 1e-5, "cavity f")
+    r1, u1 = O.get_macroscopic(to_np(got).astype(np.float64))
+    r2, u2 = O.get_macroscopic(ref)
+    assert rel_err(u1, u2) < 1e-4
+    say(f"cavity rel err f {e:.2e}")
+
+
+@pytest.mark.parametrize("storage,tol", [("f16s", 2e-3), ("bf16s", 2e-2)])
+def test_half_storage(sim, storage, tol):
+    nx, ny, n, om = 40, 32, 30, 1.7
+    f0 = rand_state(nx, ny, u0=0.03)
+    walls = WALLSETS["viv_like"]
+    got = sim.LBM((nx, ny), om, collision="mrt", walls=walls, storage=storage).run(dev(f0), n)
+    ref = run_oracle(f0, n, om, walls, "mrt", None, None)
+    e = assert_close(got, ref, tol, storage)
+    say(f"{storage}: rel err {e:.2e}")
+
+
+# ------------------------------------------------------------------ VIV (IB-LBM)
+
+def small_case(cls, **kw):
+    base = dict(nx=160, ny=120, D=16.0, cx=48.0, cy=60.0, n_markers=64, U0=0.05, Re=60.0, Ur=5.0, mass_ratio=2.0,
+                n_iter=3, pad=6, collision="mrt", forcing="guo", dof=(False, True))
+    base.update(kw)
+    return cls(**base)
+
+
+@pytest.mark.parametrize("variant", [dict(), dict(collision="kbc", forcing="edm"), dict(collision="reg", forcing="edm"),
+                                     dict(collision="bgk", forcing="guo", dof=(True, True), kernel="cosine_4pt"),
+                                     dict(kernel="peskin_3pt", n_iter=1, inlet="fc_nebb", outlet="eq")],
+                         ids=["mrt_guo", "kbc_edm", "reg_edm", "bgk_guo_2dof_cos", "p3_nebb"])
+def test_viv_matches_oracle(sim, variant):
+    oc = small_case(S.VIV2D, **variant)
+    pc = small_case(sim.VIVCase, **variant)
+    steps = 40
+    f0 = oc.initial_f(np.float64)
+    v0 = np.array([0.0, 1e-3])
+    solver = sim.VIV2D(pc)
+    solver.set_body(v=v0)
+    say(f"viv {variant}")
+    got, hd, hh = solver.run(dev(f0), steps)
+    f, d, v, a, dh, hhist = S.viv_run(oc, f0, steps, v=v0)
+    assert_close(got, f, 5e-5, "viv f")
+    assert_close(hh, hhist, 1e-4, "viv h history")
+    assert_close(hd, dh, 1e-4, "viv d history")
+    a_, v_, d_ = solver.state()
+    assert np.allclose(d_, d, rtol=1e-4, atol=1e-12)
+
+
+def test_viv_zero_iterations_is_plain_step(sim):
+    """With n_iter = 0 the IB adds nothing: the VIV run equals the plain fused run."""
+    pc = small_case(sim.VIVCase, n_iter=0)
+    f0 = dev(small_case(S.VIV2D).initial_f(np.float32))
+    a, _, hh = sim.VIV2D(pc).run(f0, 10)
+    b = sim.LBM((pc.nx, pc.ny), pc.omega, collision=pc.collision, walls=pc.walls(), forcing=pc.forcing).run(f0, 10)
+    assert torch.equal(a, b)
+    assert torch.all(hh == 0)

@@ -1,0 +1,251 @@
+"""Fused time-step drivers.
+
+The reference leaves the time step to the user, who composes the pure
+functions and compiles a chunk of steps with ``jax.jit`` + ``lax.scan``
+(README.md:105-121; examples/2d/vortex_induced_vibration.py:96-157).  Here the
+same compositions run as fused HIP passes over device-resident buffers:
+
+* ``LBM`` — macroscopic -> equilibrium -> collision -> forcing -> streaming ->
+  ordered walls, for D2Q9 or D3Q19 (``vsx_d2q9_run`` / ``vsx_d3q19_run``).
+* ``VIV2D`` — the 2D IB-LBM vortex-induced-vibration step with multi-direct
+  forcing, FP64 body-force reduction and FP64 Newmark (``vsx_viv2d_run``).
+
+Wall lists use the dictionaries ``{"kind": "nee"|"nebb"|"eq"|"bb"|"specular",
+"mode": "plain"|"velocity"|"pressure"|"force_corrected", "loc": ...,
+"rho_wall": .., "ux_wall": .., ...}``, applied in list order after streaming
+exactly like consecutive ``boundary_*`` calls.
+"""
+
+import ctypes
+import math
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from ._util import WallBuilder, f32, numel, require, wall_nodes_shape
+
+_STORE_DTYPE = {"f32": torch.float32, "f16s": torch.float16, "bf16s": torch.bfloat16}
+
+MRT_RATES_2D = lambda om: [0, 1.4, 1.4, 0, 1.2, 0, 1.2, om, om]                          # lbm/collision/mrt.py:44
+MRT_RATES_3D = lambda om: [0, 0, 0, 0, 1.1] + [om] * 5 + [1.2] * 6 + [1.4] * 3          # lbm3d/collision/mrt.py:92-113
+
+
+class LBM:
+    """A fused LBM step configuration for a grid of ``shape`` (2 or 3 axes)."""
+
+    def __init__(self, shape, omega, collision="bgk", walls=(), forcing=None, g=None, g_uniform=None,
+                 storage="f32", mrt_rates=None):
+        shape = tuple(int(s) for s in shape)
+        require(len(shape) in (2, 3), "shape must have 2 or 3 axes")
+        self.shape, self.dim = shape, len(shape)
+        self.q = 9 if self.dim == 2 else 19
+        self.pfx = "d2q9" if self.dim == 2 else "d3q19"
+        self.storage = storage
+        p = L.VsxLbm()
+        p.nx, p.ny = shape[0], shape[1]
+        p.nz = shape[2] if self.dim == 3 else 1
+        p.collision = L.COLLISION[collision]
+        p.forcing = L.FORCING[forcing]
+        p.storage = L.STORE[storage]
+        p.omega = float(omega)
+        rates = mrt_rates if mrt_rates is not None else (MRT_RATES_2D if self.dim == 2 else MRT_RATES_3D)(float(omega))
+        for i, r in enumerate(rates):
+            p.mrt_rates[i] = float(r)
+        self._wb = WallBuilder()
+        require(len(walls) <= (4 if self.dim == 2 else 6), "too many walls")
+        locs = ("left", "right", "top", "bottom") if self.dim == 2 else ("left", "right", "bottom", "top", "back", "front")
+        p.n_walls = len(walls)
+        for k, w in enumerate(walls):
+            if w["loc"] not in locs:
+                raise KeyError(w["loc"])
+            vals = {"rho": w.get("rho_wall", 1.0), "ux": w.get("ux_wall", 0.0), "uy": w.get("uy_wall", 0.0),
+                    "uz": w.get("uz_wall", 0.0), "gx": w.get("gx_wall", 0.0), "gy": w.get("gy_wall", 0.0),
+                    "gz": w.get("gz_wall", 0.0)}
+            p.walls[k] = self._wb(w["kind"], w.get("mode", "plain"), w["loc"], wall_nodes_shape(shape, w["loc"]), **vals)
+        self._g = None
+        if g is not None:
+            self._g = f32(g)
+            require(tuple(self._g.shape) == (self.dim,) + shape, "g must have shape (D, *grid)")
+            p.g = self._g.data_ptr()
+        for a, v in enumerate(g_uniform or ()):
+            p.g_uniform[a] = float(v)
+        self.p = p
+        self._work = None
+
+    @property
+    def cells(self):
+        return numel(self.shape)
+
+    def work(self):
+        if self._work is None:
+            dt = _STORE_DTYPE[self.storage]
+            self._work = [torch.empty(self.q * self.cells, dtype=dt, device=L.device()) for _ in range(2)]
+        return self._work
+
+    def run(self, f, n_steps, out=None):
+        """f_out = step^n_steps(f) for a post-streaming field f[Q, *grid] (float32)."""
+        f = f32(f)
+        require(tuple(f.shape) == (self.q,) + self.shape, f"f must have shape {(self.q,) + self.shape}")
+        out = torch.empty_like(f) if out is None else out
+        wa, wb = self.work()
+        L.call(f"vsx_{self.pfx}_run", ctypes.byref(self.p), L.ptr(f), L.ptr(out), L.ptr(wa), L.ptr(wb), int(n_steps))
+        return out
+
+    def step(self, src, dst, phase, src_f32=False, dst_f32=False):
+        """One pass of the pull scheme (phase 'collide' | 'full' | 'stream')."""
+        L.call(f"vsx_{self.pfx}_step", ctypes.byref(self.p), L.ptr(src), int(src_f32), L.ptr(dst), int(dst_f32),
+               L.PHASE[phase])
+
+
+# ====================================================================== VIV 2D
+
+@dataclass
+class VIVCase:
+    """2D vortex-induced vibration of a cylinder on springs
+    (examples/2d/vortex_induced_vibration.py:31-74 for the geometry / structure
+    definitions).  ``dof`` selects the moving DOFs (x, y)."""
+    nx: int = 1024
+    ny: int = 1024
+    D: float = 40.0
+    cx: float = 256.0
+    cy: float = 512.0
+    n_markers: int = 512
+    U0: float = 0.05
+    Re: float = 100.0
+    Ur: float = 5.0
+    mass_ratio: float = 2.0
+    damping_ratio: float = 0.0
+    n_iter: int = 3
+    pad: int = 10
+    collision: str = "mrt"
+    forcing: str = "guo"
+    kernel: str = "peskin_4pt"
+    dof: tuple = (False, True)
+    inlet: str = "eq"
+    outlet: str = "pressure_nee"
+    seed: int = 0
+    perturb: float = 0.01
+
+    @property
+    def nu(self):
+        return self.U0 * self.D / self.Re
+
+    @property
+    def omega(self):
+        return 1 / (3 * self.nu + 0.5)
+
+    @property
+    def area(self):
+        return math.pi * (self.D / 2) ** 2
+
+    @property
+    def mass(self):
+        return self.area * self.mass_ratio
+
+    @property
+    def stiffness(self):
+        fn = self.U0 / (self.Ur * self.D)
+        return (2 * math.pi * fn) ** 2 * self.mass * (1 + 1 / self.mass_ratio)
+
+    @property
+    def damping(self):
+        return 2 * math.sqrt(self.stiffness * self.mass) * self.damping_ratio
+
+    @property
+    def ib_x0(self):
+        return int(self.cx - 0.5 * self.D - self.pad)
+
+    @property
+    def ib_y0(self):
+        return int(self.cy - 0.5 * self.D - self.pad)
+
+    @property
+    def ib_size(self):
+        return int(self.D + 2 * self.pad)
+
+    @property
+    def ds(self):
+        return math.pi * self.D / self.n_markers
+
+    def markers(self):
+        th = 2 * np.pi * np.arange(self.n_markers) / self.n_markers
+        return self.cx + 0.5 * self.D * np.cos(th), self.cy + 0.5 * self.D * np.sin(th)
+
+    def walls(self):
+        w = []
+        if self.inlet == "eq":
+            w.append(dict(kind="eq", mode="plain", loc="left", ux_wall=self.U0))
+        elif self.inlet == "fc_nebb":
+            w.append(dict(kind="nebb", mode="force_corrected", loc="left", ux_wall=self.U0))
+        if self.outlet == "pressure_nee":
+            w.append(dict(kind="nee", mode="pressure", loc="right", rho_wall=1.0))
+        elif self.outlet == "eq":
+            w.append(dict(kind="eq", mode="plain", loc="right", ux_wall=self.U0))
+        return w
+
+    def initial_u(self):
+        """u = U0 (1 + eps_x, eps_y), eps ~ U(-p, p) from default_rng(seed) (float64, host)."""
+        rng = np.random.default_rng(self.seed)
+        eps = rng.uniform(-self.perturb, self.perturb, size=(2, self.nx, self.ny))
+        u = np.empty((2, self.nx, self.ny))
+        u[0] = self.U0 * (1 + eps[0])
+        u[1] = self.U0 * eps[1]
+        return u
+
+
+class VIV2D:
+    """Device state and fused run of a ``VIVCase``."""
+
+    def __init__(self, case: VIVCase, storage="f32"):
+        self.case = case
+        dev = L.device()
+        self.lbm = LBM((case.nx, case.ny), case.omega, collision=case.collision, walls=case.walls(),
+                       forcing=case.forcing, storage=storage)
+        mx, my = case.markers()
+        self.marker_x = torch.as_tensor(mx, dtype=torch.float64, device=dev)
+        self.marker_y = torch.as_tensor(my, dtype=torch.float64, device=dev)
+        S = case.ib_size
+        self.g_win = torch.zeros(2 * S * S, dtype=torch.float32, device=dev)
+        self.win_origin = torch.zeros(2, dtype=torch.int32, device=dev)
+        self.body = torch.zeros(6, dtype=torch.float64, device=dev)     # a0 a1 v0 v1 d0 d1
+        ib = L.VsxIB2D()
+        ib.n_markers = case.n_markers
+        ib.kernel = L.KERNEL[case.kernel]
+        ib.n_iter = case.n_iter
+        ib.win = S
+        ib.marker_x, ib.marker_y = self.marker_x.data_ptr(), self.marker_y.data_ptr()
+        ib.ds_arr = None
+        ib.ds = case.ds
+        ib.dof_mask = int(bool(case.dof[0])) | (int(bool(case.dof[1])) << 1)
+        ib.win_x0, ib.win_y0 = float(case.ib_x0), float(case.ib_y0)
+        ib.mass, ib.stiffness, ib.damping, ib.area = case.mass, case.stiffness, case.damping, case.area
+        ib.dt, ib.gamma, ib.beta = 1.0, 0.5, 0.25
+        ib.body = self.body.data_ptr()
+        ib.g_win = self.g_win.data_ptr()
+        ib.win_origin = self.win_origin.data_ptr()
+        self.ib = ib
+        self._hist = None
+
+    def set_body(self, a=(0.0, 0.0), v=(0.0, 0.0), d=(0.0, 0.0)):
+        self.body.copy_(torch.tensor(list(a) + list(v) + list(d), dtype=torch.float64))
+
+    def state(self):
+        b = self.body.cpu().numpy()
+        return b[0:2].copy(), b[2:4].copy(), b[4:6].copy()   # a, v, d
+
+    def run(self, f, n_steps, out=None, history=True):
+        """Advance n_steps; returns (f_out, d_hist [n,2], h_hist [n,2]) (histories FP64 on device)."""
+        f = f32(f)
+        out = torch.empty_like(f) if out is None else out
+        hd = hh = None
+        if history and n_steps > 0:
+            hd = torch.empty((n_steps, 2), dtype=torch.float64, device=f.device)
+            hh = torch.empty((n_steps, 2), dtype=torch.float64, device=f.device)
+        self.ib.hist_d = None if hd is None else hd.data_ptr()
+        self.ib.hist_h = None if hh is None else hh.data_ptr()
+        wa, wb = self.lbm.work()
+        L.call("vsx_viv2d_run", ctypes.byref(self.lbm.p), ctypes.byref(self.ib), L.ptr(f), L.ptr(out), L.ptr(wa),
+               L.ptr(wb), int(n_steps))
+        return out, hd, hh
