@@ -16,11 +16,16 @@
 //
 // Wall cells: the reference applies each wall to the whole streamed array, in
 // order, so a later wall can read cells an earlier wall rewrote (NEE reads the
-// neighbour line) and the last writer wins at corners.  Cells on a wall line
-// are therefore handled by a separate edge kernel that replays the wall list
-// recursively (depth <= number of walls); the bulk kernel handles all other
-// cells with a plain pull and no wall logic.
+// neighbour line) and the last writer wins at corners.  A cell on exactly one
+// wall whose update reads no cell rewritten by an earlier wall is finished in
+// the bulk kernel (one extra pull for the neighbour line).  The remaining cells
+// (corners and corner-adjacent cells of perpendicular wall pairs) go to a
+// separate edge kernel that replays the wall list recursively (depth <= number
+// of walls); it is launched only when such cells can exist.
 #pragma once
+#include <type_traits>
+#include <utility>
+
 #include "cell.hpp"
 #include "common.hpp"
 #include "lattice.hpp"
@@ -40,6 +45,11 @@ struct StepCtx {
   Window win;
   int32_t dims[3];         // nx, ny, nz (nz = 1 in 2D)
   int64_t n;               // cells
+  // Walls by location, so device code indexes walls only by compile-time
+  // location: order[LOC] = position of the wall at LOC in p.walls (-1: none,
+  // -2: several walls at LOC), at[LOC] = that wall.
+  int32_t order[6];
+  VsxWall at[6];
 };
 
 template <class L>
@@ -68,10 +78,22 @@ __device__ __forceinline__ void local(const StepCtx &c, const typename SRC::T *_
   for (int i = 0; i < L::Q; ++i) f[i] = SRC::ld(src, (size_t)i * c.n + cell, L::w[i]);
 }
 
+// component a of a coordinate triple, by selects (no dynamic register indexing)
+__device__ __forceinline__ int comp3(const int (&v)[3], int a) { return a == 0 ? v[0] : (a == 1 ? v[1] : v[2]); }
+
+// coordinates of the inward neighbour across wall `loc`
+template <class L>
+__device__ __forceinline__ void inward(int loc, const int (&co)[3], int (&cn)[3]) {
+  const int a = L::w_axis[loc], sg = L::w_sign[loc];
+  cn[0] = co[0] + (a == 0 ? sg : 0);
+  cn[1] = co[1] + (a == 1 ? sg : 0);
+  cn[2] = co[2] + (a == 2 ? sg : 0);
+}
+
 template <class L>
 __device__ __forceinline__ bool on_wall(const StepCtx &c, int loc, const int (&co)[3]) {
   const int a = L::w_axis[loc];
-  return co[a] == (L::w_sign[loc] > 0 ? 0 : c.dims[a] - 1);
+  return comp3(co, a) == (L::w_sign[loc] > 0 ? 0 : comp3(c.dims, a) - 1);
 }
 
 // index of a cell along the wall (matches the per-node wall arrays)
@@ -86,7 +108,9 @@ __device__ __forceinline__ int64_t wall_node(const StepCtx &c, int loc, const in
 template <class L>
 __device__ __forceinline__ bool on_any_wall(const StepCtx &c, const int (&co)[3], int upto) {
   bool on = false;
-  for (int k = 0; k < upto; ++k) on |= on_wall<L>(c, c.p.walls[k].loc, co);
+#pragma unroll
+  for (int k = 0; k < L::n_walls; ++k)
+    if (k < upto) on |= on_wall<L>(c, c.p.walls[k].loc, co);
   return on;
 }
 
@@ -105,8 +129,8 @@ __device__ __noinline__ void wall_state(const StepCtx &c, const typename SRC::T 
 #pragma unroll
         for (int i = 0; i < L::Q; ++i) fn[i] = fp[i] = 0.f;
         if (wall_needs_neighbor(w)) {
-          int cn[3] = {co[0], co[1], co[2]};
-          cn[L::w_axis[w.loc]] += L::w_sign[w.loc];
+          int cn[3];
+          inward<L>(w.loc, co, cn);
           wall_state<L, SRC, J - 1>(c, src, cn, fn);
         }
         if (wall_needs_pre(w)) local<L, SRC>(c, src, cell_of<L>(c, co[0], co[1], co[2]), fp);
@@ -114,6 +138,96 @@ __device__ __noinline__ void wall_state(const StepCtx &c, const typename SRC::T 
       }
     }
   }
+}
+
+// Calls f(std::integral_constant<int, I>) for I = 0 .. N-1 (compile-time loop).
+template <class F, int... I>
+__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F &&f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+template <class L, int LOC>
+__device__ __forceinline__ bool on_wall_at(const StepCtx &c, const int (&co)[3]) {
+  constexpr int a = L::w_axis[LOC];
+  return co[a] == (L::w_sign[LOC] > 0 ? 0 : c.dims[a] - 1);
+}
+
+// Wall class of a cell: -1 on no wall; LOC when the cell is on exactly one wall
+// (at location LOC) and that wall reads no neighbour rewritten by an earlier
+// wall, so one pull and one wall update reproduce the ordered sequence; -2
+// otherwise (corners and corner-adjacent cells, replayed recursively by the
+// edge kernel).
+template <class L>
+__device__ __forceinline__ int wall_class(const StepCtx &c, const int (&co)[3]) {
+  int loc_on = -1, count = 0;
+  static_for<L::n_walls>([&](auto LC) {
+    constexpr int LOC = decltype(LC)::value;
+    if (c.order[LOC] != -1 && on_wall_at<L, LOC>(c, co)) {
+      loc_on = LOC;
+      count += c.order[LOC] == -2 ? 2 : 1;
+    }
+  });
+  if (count == 0) return -1;
+  if (count > 1) return -2;
+  int result = loc_on;
+  static_for<L::n_walls>([&](auto LC) {
+    constexpr int LOC = decltype(LC)::value;
+    if (LOC == loc_on && wall_needs_neighbor(c.at[LOC])) {
+      int cn[3];
+      inward<L>(LOC, co, cn);
+      static_for<L::n_walls>([&](auto L2C) {
+        constexpr int L2 = decltype(L2C)::value;
+        if (c.order[L2] >= 0 && c.order[L2] < c.order[LOC] && on_wall_at<L, L2>(c, cn)) result = -2;
+      });
+    }
+  });
+  return result;
+}
+
+// Populations of a cell of wall class cls >= 0 (the wall location).
+template <class L, class SRC>
+__device__ __forceinline__ void single_wall_state(const StepCtx &c, const typename SRC::T *__restrict__ src,
+                                                  const int (&co)[3], int cls, float (&f)[L::Q]) {
+  pull<L, SRC>(c, src, co[0], co[1], co[2], f);
+  static_for<L::n_walls>([&](auto LC) {
+    constexpr int LOC = decltype(LC)::value;
+    if (LOC != cls) return;
+    const VsxWall &w = c.at[LOC];
+    float fn[L::Q], fp[L::Q];
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i) fn[i] = fp[i] = 0.f;
+    if (wall_needs_neighbor(w)) {
+      int cn[3];
+      inward<L>(LOC, co, cn);
+      pull<L, SRC>(c, src, cn[0], cn[1], cn[2], fn);
+    }
+    if (wall_needs_pre(w)) local<L, SRC>(c, src, cell_of<L>(c, co[0], co[1], co[2]), fp);
+    apply_wall_at<L, LOC>(w, wall_node<L>(c, LOC, co), f, fn, fp);
+  });
+}
+
+// Post-streaming, post-walls populations of any cell.
+template <class L, class SRC>
+__device__ __forceinline__ void cell_state(const StepCtx &c, const typename SRC::T *__restrict__ src, const int (&co)[3],
+                                           float (&f)[L::Q]) {
+  const int cls = c.p.n_walls > 0 ? wall_class<L>(c, co) : -1;
+  if (cls == -1) pull<L, SRC>(c, src, co[0], co[1], co[2], f);
+  else if (cls >= 0) single_wall_state<L, SRC>(c, src, co, cls, f);
+  else wall_state<L, SRC, L::n_walls>(c, src, co, f);
+}
+
+// Whether any cell can be of class -2: two walls on different axes (shared
+// corners) or two walls at the same location.
+template <class L>
+inline bool walls_need_edge(const VsxLbm &p) {
+  for (int i = 0; i < p.n_walls; ++i)
+    for (int j = i + 1; j < p.n_walls; ++j)
+      if (L::w_axis[p.walls[i].loc] != L::w_axis[p.walls[j].loc] || p.walls[i].loc == p.walls[j].loc) return true;
+  return false;
 }
 
 // body force at a cell: uniform + optional full field + immersed-boundary window
@@ -237,15 +351,24 @@ __global__ __launch_bounds__(256) void k_bulk(StepCtx c, const typename SRC::T *
   if constexpr (PHASE == VSX_PHASE_COLLIDE) {
     local<L, SRC>(c, src, cell, f);
   } else {
-    if (on_any_wall<L>(c, co, c.p.n_walls)) return;   // edge kernel
+    const int cls = c.p.n_walls > 0 ? wall_class<L>(c, co) : -1;
+    if (cls != -1) {
+      // wall cell: a separate path (own finish and store) keeps the wall code's
+      // register demand out of the interior path below
+      if (cls < 0) return;   // corner cells: edge kernel
+      single_wall_state<L, SRC>(c, src, co, cls, f);
+      if constexpr (PHASE != VSX_PHASE_STREAM) finish<L, COLL>(c, cell, co, f);
+      store_cell<L, DST>(c, dst, cell, f);
+      return;
+    }
     pull<L, SRC>(c, src, co[0], co[1], co[2], f);
   }
   if constexpr (PHASE != VSX_PHASE_STREAM) finish<L, COLL>(c, cell, co, f);
   store_cell<L, DST>(c, dst, cell, f);
 }
 
-// Edge: one thread per (wall, node); a cell on several walls is handled by the
-// first wall (in list order) that contains it.
+// Edge: the cells of wall class -2, enumerated as (wall, node) pairs; a cell on
+// several walls is handled under the first wall (in list order) that contains it.
 template <class L, int PHASE, class SRC, class DST>
 __global__ __launch_bounds__(256) void k_edge(StepCtx c, const typename SRC::T *__restrict__ src,
                                               typename DST::T *__restrict__ dst, int64_t per_wall) {
@@ -269,7 +392,8 @@ __global__ __launch_bounds__(256) void k_edge(StepCtx c, const typename SRC::T *
     co[b0] = (int)(j / n1);
     co[b1] = (int)(j % n1);
   }
-  if (on_any_wall<L>(c, co, k)) return;
+  if (on_any_wall<L>(c, co, k)) return;     // handled under the first wall that contains it
+  if (wall_class<L>(c, co) != -2) return;   // bulk kernel
   const int64_t cell = cell_of<L>(c, co[0], co[1], co[2]);
   float f[L::Q];
   wall_state<L, SRC, L::n_walls>(c, src, co, f);

@@ -54,20 +54,29 @@ __global__ __launch_bounds__(kIbThreads) void k_viv_ib(StepCtx c, VsxIB2D ib, co
   __syncthreads();
   const int x0 = s_org[0], y0 = s_org[1], xc = s_org[2], yc = s_org[3];
 
-  // window velocity
-  for (int k = tid; k < SS; k += nt) {
-    int co[3] = {xc + k / S, yc + k % S, 0};
-    float f[9];
-    if constexpr (!PULL) {
-      local<D2Q9, SRC>(c, src, cell_of<D2Q9>(c, co[0], co[1], 0), f);
-    } else {
-      if (on_any_wall<D2Q9>(c, co, c.p.n_walls)) wall_state<D2Q9, SRC, D2Q9::n_walls>(c, src, co, f);
-      else pull<D2Q9, SRC>(c, src, co[0], co[1], 0, f);
+  // window velocity: kUW cells per thread per pass, all loads issued before use
+  constexpr int kUW = 4;
+  for (int base = 0; base < SS; base += kUW * nt) {
+    float f[kUW][9];
+#pragma unroll
+    for (int j = 0; j < kUW; ++j) {
+      const int k = base + j * nt + tid;
+      if (k < SS) {
+        const int co[3] = {xc + k / S, yc + k % S, 0};
+        if constexpr (!PULL) local<D2Q9, SRC>(c, src, cell_of<D2Q9>(c, co[0], co[1], 0), f[j]);
+        else cell_state<D2Q9, SRC>(c, src, co, f[j]);
+      }
     }
-    float rho, u[3];
-    macroscopic<D2Q9>(f, rho, u);
-    ul[k] = u[0];
-    ul[SS + k] = u[1];
+#pragma unroll
+    for (int j = 0; j < kUW; ++j) {
+      const int k = base + j * nt + tid;
+      if (k < SS) {
+        float rho, u[3];
+        macroscopic<D2Q9>(f[j], rho, u);
+        ul[k] = u[0];
+        ul[SS + k] = u[1];
+      }
+    }
   }
 
   // marker stencils and initial interpolation
@@ -76,9 +85,16 @@ __global__ __launch_bounds__(kIbThreads) void k_viv_ib(StepCtx c, VsxIB2D ib, co
   float ubx[MPT], uby[MPT], Fx[MPT], Fy[MPT];
   float dFx[MPT], dFy[MPT], ds2[MPT];
   __syncthreads();
+  // Marker q of this thread: consecutive lanes of a wavefront hold markers nt/64
+  // apart along the body, so the 64 lanes of one LDS atomic (spread) or read
+  // (interpolate) touch different cells instead of serialising on shared ones.
+  int mk[MPT];
+  const int nwave = nt >> 6;
+#pragma unroll
+  for (int q = 0; q < MPT; ++q) mk[q] = q * nt + (tid & 63) * nwave + (tid >> 6);
 #pragma unroll
   for (int q = 0; q < MPT; ++q) {
-    const int m = tid + q * nt;
+    const int m = mk[q];
     ubx[q] = uby[q] = Fx[q] = Fy[q] = dFx[q] = dFy[q] = ds2[q] = 0.f;
     if (m < ib.n_markers) {
       const float rx = (float)(ib.marker_x[m] + s_body[4] - (double)x0);
@@ -120,7 +136,7 @@ __global__ __launch_bounds__(kIbThreads) void k_viv_ib(StepCtx c, VsxIB2D ib, co
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < MPT; ++q) {
-      if (tid + q * nt < ib.n_markers) {
+      if (mk[q] < ib.n_markers) {
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -138,7 +154,7 @@ __global__ __launch_bounds__(kIbThreads) void k_viv_ib(StepCtx c, VsxIB2D ib, co
     if (last) break;
 #pragma unroll
     for (int q = 0; q < MPT; ++q) {
-      if (tid + q * nt < ib.n_markers) {
+      if (mk[q] < ib.n_markers) {
         float sx = 0.f, sy = 0.f;
 #pragma unroll
         for (int i = 0; i < 4; ++i)

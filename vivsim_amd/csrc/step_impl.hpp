@@ -33,6 +33,15 @@ StepCtx make_ctx(const VsxLbm &p, Window win) {
   c.dims[1] = p.ny;
   c.dims[2] = L::D == 3 ? p.nz : 1;
   c.n = (int64_t)c.dims[0] * c.dims[1] * c.dims[2];
+  for (int l = 0; l < 6; ++l) {
+    c.order[l] = -1;
+    c.at[l] = VsxWall{};
+  }
+  for (int k = 0; k < p.n_walls; ++k) {
+    const int l = p.walls[k].loc;
+    c.order[l] = c.order[l] == -1 ? k : -2;
+    c.at[l] = p.walls[k];
+  }
   return c;
 }
 
@@ -78,7 +87,7 @@ int launch_phase(const StepCtx &c, const void *src, void *dst, hipStream_t s) {
   }
   if constexpr (PHASE == VSX_PHASE_FULL) timer_end(s);
   if constexpr (PHASE != VSX_PHASE_COLLIDE) {
-    if (c.p.n_walls > 0) {
+    if (c.p.n_walls > 0 && walls_need_edge<L>(c.p)) {
       const int64_t pw = edge_per_wall<L>(c);
       const int64_t tot = pw * c.p.n_walls;
       hipLaunchKernelGGL((k_edge<L, PHASE, SRC, DST>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, c, S, D, pw);

@@ -31,9 +31,9 @@ __host__ __device__ inline bool wall_needs_pre(const VsxWall &w) {
 
 // sum of wall-parallel populations + 2 * sum of outgoing ones
 // (lbm/boundary/_helpers.py:135-145, lbm3d/boundary/_helpers.py:35-42)
-template <class L>
-__device__ __forceinline__ float rho_wall_numerator(int loc, const float (&f)[L::Q]) {
-  const int a = L::w_axis[loc], sg = L::w_sign[loc];
+template <class L, int LOC>
+__device__ __forceinline__ float rho_wall_numerator(const float (&f)[L::Q]) {
+  constexpr int a = L::w_axis[LOC], sg = L::w_sign[LOC];
   float z = 0.f, o = 0.f;
 #pragma unroll
   for (int i = 0; i < L::Q; ++i) {
@@ -45,10 +45,10 @@ __device__ __forceinline__ float rho_wall_numerator(int loc, const float (&f)[L:
 }
 
 // Resolve (rho_w, u_w) at wall node j for the wrapper mode of the wall.
-template <class L>
-__device__ __forceinline__ void wall_state(const VsxWall &w, int64_t j, const float (&f)[L::Q],
-                                           const float (&fn)[L::Q], float &rho, float (&u)[3]) {
-  const int a = L::w_axis[w.loc], sg = L::w_sign[w.loc];
+template <class L, int LOC>
+__device__ __forceinline__ void wall_state(const VsxWall &w, int64_t j, const float (&f)[L::Q], const float (&fn)[L::Q],
+                                           float &rho, float (&u)[3]) {
+  constexpr int a = L::w_axis[LOC], sg = L::w_sign[LOC];
   rho = wv(w.rho_arr, w.rho, j);
   u[0] = wv(w.ux_arr, w.ux, j);
   u[1] = wv(w.uy_arr, w.uy, j);
@@ -61,7 +61,7 @@ __device__ __forceinline__ void wall_state(const VsxWall &w, int64_t j, const fl
   } else if (w.mode == VSX_MODE_VELOCITY) {
     // rho = numerator / (1 - u_n)   (_helpers.py:80-95)
     const float un = sg * u[a];
-    rho = rho_wall_numerator<L>(w.loc, f) / (1.f - un);
+    rho = rho_wall_numerator<L, LOC>(f) / (1.f - un);
   } else if (w.mode == VSX_MODE_PRESSURE) {
     // u_n from the prescribed density, tangential velocity of the neighbour (_helpers.py:98-132)
     float rn, un_b[3];
@@ -69,30 +69,26 @@ __device__ __forceinline__ void wall_state(const VsxWall &w, int64_t j, const fl
     u[0] = un_b[0];
     u[1] = un_b[1];
     u[2] = un_b[2];
-    u[a] = sg * (1.f - rho_wall_numerator<L>(w.loc, f) / rho);
+    u[a] = sg * (1.f - rho_wall_numerator<L, LOC>(f) / rho);
   }
 }
 
-template <class L>
-__device__ __forceinline__ void apply_wall(const VsxWall &w, int64_t j, float (&f)[L::Q],
-                                           const float (&fn)[L::Q], const float (&fpre)[L::Q]) {
-  const int loc = w.loc, a = L::w_axis[loc], sg = L::w_sign[loc];
+template <class L, int LOC>
+__device__ __forceinline__ void apply_wall_at(const VsxWall &w, int64_t j, float (&f)[L::Q], const float (&fn)[L::Q],
+                                              const float (&fpre)[L::Q]) {
+  constexpr int a = L::w_axis[LOC], sg = L::w_sign[LOC];
   if (w.kind == VSX_WALL_BB || w.kind == VSX_WALL_SPECULAR) {
     float uw[3] = {wv(w.ux_arr, w.ux, j), wv(w.uy_arr, w.uy, j), wv(w.uz_arr, w.uz, j)};
     if constexpr (L::D == 2) {
       // in_k := pre[out_k] + {2/3 u_n, (u_n + u_t)/6, (u_n - u_t)/6}; specular swaps the diagonal targets
       const float un = sg * uw[a], ut = sg * uw[1 - a];
-      const float v0 = fpre[L::w_out[loc][0]] + 2.f / 3.f * un;
-      const float v1 = fpre[L::w_out[loc][1]] + 1.f / 6.f * (un + ut);
-      const float v2 = fpre[L::w_out[loc][2]] + 1.f / 6.f * (un - ut);
-      f[L::w_in[loc][0]] = v0;
-      if (w.kind == VSX_WALL_BB) {
-        f[L::w_in[loc][1]] = v1;
-        f[L::w_in[loc][2]] = v2;
-      } else {
-        f[L::w_in[loc][2]] = v1;
-        f[L::w_in[loc][1]] = v2;
-      }
+      const float v0 = fpre[L::w_out[LOC][0]] + 2.f / 3.f * un;
+      const float v1 = fpre[L::w_out[LOC][1]] + 1.f / 6.f * (un + ut);
+      const float v2 = fpre[L::w_out[LOC][2]] + 1.f / 6.f * (un - ut);
+      f[L::w_in[LOC][0]] = v0;
+      const bool bb = w.kind == VSX_WALL_BB;
+      f[L::w_in[LOC][1]] = bb ? v1 : v2;
+      f[L::w_in[LOC][2]] = bb ? v2 : v1;
     } else {
       // in_k := pre[out_k] + 2 w_in rho_pre (c_in . u_w) / cs2 (bounce-back and specular share the body)
       float rho = 0.f;
@@ -100,15 +96,15 @@ __device__ __forceinline__ void apply_wall(const VsxWall &w, int64_t j, float (&
       for (int i = 0; i < L::Q; ++i) rho += fpre[i];
 #pragma unroll
       for (int k = 0; k < L::n_in; ++k) {
-        const int i = L::w_in[loc][k];
-        f[i] = fpre[L::w_out[loc][k]] + 2.0f * L::w[i] * rho * cdot<L>(i, uw) * 3.0f;
+        const int i = L::w_in[LOC][k];
+        f[i] = fpre[L::w_out[LOC][k]] + 2.0f * L::w[i] * rho * cdot<L>(i, uw) * 3.0f;
       }
     }
     return;
   }
 
   float rho, u[3];
-  wall_state<L>(w, j, f, fn, rho, u);
+  wall_state<L, LOC>(w, j, f, fn, rho, u);
 
   if (w.kind == VSX_WALL_EQ) {
     equilibrium<L>(rho, u, f);
@@ -123,23 +119,42 @@ __device__ __forceinline__ void apply_wall(const VsxWall &w, int64_t j, float (&
   } else {  // NEBB
     if constexpr (L::D == 2) {
       const float un = sg * u[a], ut = sg * u[1 - a];
-      const float shear = 0.5f * (f[L::w_tan[loc][0]] - f[L::w_tan[loc][1]]) * (float)sg;
+      const float shear = 0.5f * (f[L::w_tan[LOC][0]] - f[L::w_tan[LOC][1]]) * (float)sg;
       const float nt = 1.f / 6.f * un * rho, tt = 0.5f * ut * rho;
-      const float v0 = f[L::w_out[loc][0]] + 2.f / 3.f * un * rho;
-      const float v1 = f[L::w_out[loc][1]] - shear + nt + tt;
-      const float v2 = f[L::w_out[loc][2]] + shear + nt - tt;
-      f[L::w_in[loc][0]] = v0;
-      f[L::w_in[loc][1]] = v1;
-      f[L::w_in[loc][2]] = v2;
+      const float v0 = f[L::w_out[LOC][0]] + 2.f / 3.f * un * rho;
+      const float v1 = f[L::w_out[LOC][1]] - shear + nt + tt;
+      const float v2 = f[L::w_out[LOC][2]] + shear + nt - tt;
+      f[L::w_in[LOC][0]] = v0;
+      f[L::w_in[LOC][1]] = v1;
+      f[L::w_in[LOC][2]] = v2;
     } else {
       // f_in := f_opp(in) + feq_in - feq_opp(in)
       const float u2 = usq<L>(u);
 #pragma unroll
       for (int k = 0; k < L::n_in; ++k) {
-        const int i = L::w_in[loc][k], o = L::opp[i];
+        const int i = L::w_in[LOC][k], o = L::opp[i];
         f[i] = f[o] + feq_i<L>(i, rho, u, u2) - feq_i<L>(o, rho, u, u2);
       }
     }
+  }
+}
+
+// Apply wall w at its node j: dispatch on the location so that every lattice
+// table lookup is a compile-time constant (no dynamic register indexing).
+template <class L>
+__device__ __forceinline__ void apply_wall(const VsxWall &w, int64_t j, float (&f)[L::Q], const float (&fn)[L::Q],
+                                           const float (&fpre)[L::Q]) {
+  switch (w.loc) {
+    case 0: apply_wall_at<L, 0>(w, j, f, fn, fpre); break;
+    case 1: apply_wall_at<L, 1>(w, j, f, fn, fpre); break;
+    case 2: apply_wall_at<L, 2>(w, j, f, fn, fpre); break;
+    case 3: apply_wall_at<L, 3>(w, j, f, fn, fpre); break;
+    default:
+      if constexpr (L::n_walls > 4) {
+        if (w.loc == 4) apply_wall_at<L, 4>(w, j, f, fn, fpre);
+        else apply_wall_at<L, 5>(w, j, f, fn, fpre);
+      }
+      break;
   }
 }
 
