@@ -10,7 +10,7 @@ step; ``value`` is GLUPS = cells * steps / seconds / 1e9 with the state
 resident in HBM.
 
 Also reported on the same line:
-  roofline      : the bulk stream-collide kernel's achieved algorithmic bandwidth
+  roofline      : the fused step kernel's achieved algorithmic bandwidth
                   (72 B per cell in FP32 / 36 B in 16-bit storage) from HIP events
                   recorded around every launch in the timed region, vs 8 TB/s
   variants      : FP16 / BF16 shifted-storage GLUPS of the same step
@@ -59,7 +59,7 @@ def run_gpu(args, storage, steps, warmup):
     assert torch.isfinite(out).all().item(), "non-finite state"
     cells = case.nx * case.ny
     sec = t1 - t0
-    bulk_cells = (case.nx - 2) * case.ny            # left / right wall columns run in the edge kernel
+    bulk_cells = case.nx * case.ny                  # every cell, wall columns included, is in the fused launch
     bpc = 72 if storage == "f32" else 36
     mean_ms = float(kt.mean())
     return dict(glups=cells * steps / sec / 1e9, ms_per_step=sec / steps * 1e3, kernel_ms=mean_ms,
@@ -143,7 +143,7 @@ def main():
                    "forcing": "guo", "kernel": "peskin_4pt", "storage": args.storage, "structure": "fp64 newmark"},
         "roofline": {"bound": "hbm", "achieved": round(main_run["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(main_run["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_bulk (pull-stream + MRT + Guo, FULL phase)",
+                     "kernel": "k_viv_fused (IB workgroup + pull-stream/MRT/Guo tiles, one launch per step)",
                      "kernel_us": round(main_run["kernel_ms"] * 1e3, 3),
                      "bytes_per_launch": main_run["bytes_per_launch"], "launches": main_run["kernel_launches"]},
         "variants": variants,

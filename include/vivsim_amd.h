@@ -98,10 +98,17 @@ typedef struct VsxIB2D {
   double win_x0, win_y0;                    /* window origin before displacement            */
   double mass, stiffness, damping, area;    /* Newmark (dyn.py:5-51), h += a * area          */
   double dt, gamma, beta;
-  double *body;                             /* device FP64 [a0 a1 v0 v1 d0 d1], in/out       */
-  float *g_win;                             /* device scratch [2][win][win]                  */
-  double *hist_d, *hist_h;                  /* optional device histories [n_steps][2]        */
-  int32_t *win_origin;                      /* device scratch [2]                            */
+  double *body;        /* device FP64 [2][6]: slot (step0 + t) & 1 holds [a0 a1 v0 v1 d0 d1]
+                          before step t; step t writes the other slot                       */
+  float *g_win;        /* device scratch [2][win][win]                                      */
+  double *hist_d, *hist_h;   /* optional device histories [n_steps][2]                      */
+  int32_t *win_origin; /* device int32 [4]: window origin x, y of the last step, ready flag
+                          (= step0 + t + 1 once step t's force window is written), error
+                          flag (1 if a wait for the ready flag ran out)                      */
+  int32_t step0;       /* global index of the first step of this run (0 after a reset of the
+                          body state, which must also zero win_origin)                       */
+  int32_t fused;       /* 1: one launch per step, the IB workgroup overlapped with the bulk
+                          pass; 0: separate IB and bulk kernels                              */
 } VsxIB2D;
 
 /* ------------------------------------------------------------------ library */

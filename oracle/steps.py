@@ -216,18 +216,29 @@ def viv_window_origin(case, d):
     return int(math.trunc(case.ib_x0 + float(d[0]))), int(math.trunc(case.ib_y0 + float(d[1])))
 
 
+def viv_window_clamp(case, x0, y0):
+    """Start of the window slice as lax.dynamic_slice / dynamic_update_slice place it:
+    the start is clamped so that the whole window lies inside the grid
+    (vortex_induced_vibration.py:108-109, 141).  The marker coordinates stay
+    relative to the unclamped origin (vortex_induced_vibration.py:116-117)."""
+    S = case.ib_size
+    return min(max(x0, 0), case.nx - S), min(max(y0, 0), case.ny - S)
+
+
 def viv_ib(case, u, d, v, dtype):
-    """IB part of the VIV step: window, stencil, MDF.  Returns (x0, y0, g_win, h_markers)."""
+    """IB part of the VIV step: window, stencil, MDF.  Returns (xc, yc, g_win, h_markers)
+    with (xc, yc) the clamped window start."""
     S = case.ib_size
     x0, y0 = viv_window_origin(case, d)
+    xc, yc = viv_window_clamp(case, x0, y0)
     mx, my = case.markers()
     rx = (mx + d[0] - x0).astype(dtype)
     ry = (my + d[1] - y0).astype(dtype)
     W, I = IB.get_ib_stencil(rx, ry, S, IB.KERNELS[case.kernel], 2)
-    ib_u = u[:, x0:x0 + S, y0:y0 + S]
+    ib_u = u[:, xc:xc + S, yc:yc + S]
     target = np.repeat(np.asarray(v, dtype=np.float64)[None, :], case.n_markers, axis=0).astype(dtype)
     g_win, h_m = IB.multi_direct_forcing(ib_u, W, I, target, np.asarray(case.ds, dtype=dtype), case.n_iter)
-    return x0, y0, g_win, h_m
+    return xc, yc, g_win, h_m
 
 
 def viv_newmark(case, a, v, d, h_markers):

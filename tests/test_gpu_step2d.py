@@ -148,24 +148,50 @@ def small_case(cls, **kw):
 
 @pytest.mark.parametrize("variant", [dict(), dict(collision="kbc", forcing="edm"), dict(collision="reg", forcing="edm"),
                                      dict(collision="bgk", forcing="guo", dof=(True, True), kernel="cosine_4pt"),
-                                     dict(kernel="peskin_3pt", n_iter=1, inlet="fc_nebb", outlet="eq")],
-                         ids=["mrt_guo", "kbc_edm", "reg_edm", "bgk_guo_2dof_cos", "p3_nebb"])
-def test_viv_matches_oracle(sim, variant):
+                                     dict(kernel="peskin_3pt", n_iter=1, inlet="fc_nebb", outlet="eq"),
+                                     dict(cx=12.0, n_markers=48), dict(cx=150.0, cy=8.0)],
+                         ids=["mrt_guo", "kbc_edm", "reg_edm", "bgk_guo_2dof_cos", "p3_nebb", "window_on_inlet",
+                              "window_on_outlet_clamped"])
+@pytest.mark.parametrize("fused", [True, False], ids=["fused", "separate"])
+def test_viv_matches_oracle(sim, variant, fused):
     oc = small_case(S.VIV2D, **variant)
     pc = small_case(sim.VIVCase, **variant)
     steps = 40
     f0 = oc.initial_f(np.float64)
     v0 = np.array([0.0, 1e-3])
-    solver = sim.VIV2D(pc)
+    solver = sim.VIV2D(pc, fused=fused)
     solver.set_body(v=v0)
-    say(f"viv {variant}")
+    say(f"viv {variant} fused={fused}")
     got, hd, hh = solver.run(dev(f0), steps)
+    solver.check()
     f, d, v, a, dh, hhist = S.viv_run(oc, f0, steps, v=v0)
     assert_close(got, f, 5e-5, "viv f")
     assert_close(hh, hhist, 1e-4, "viv h history")
     assert_close(hd, dh, 1e-4, "viv d history")
     a_, v_, d_ = solver.state()
     assert np.allclose(d_, d, rtol=1e-4, atol=1e-12)
+
+
+def test_viv_split_runs(sim):
+    """Runs of 1 + 14 + 25 steps continue the body state and the step counter
+    exactly as one run of 40 steps (the oracle's 40-step run)."""
+    oc, pc = small_case(S.VIV2D, dof=(True, True)), small_case(sim.VIVCase, dof=(True, True))
+    f0 = oc.initial_f(np.float64)
+    v0 = np.array([2e-4, 1e-3])
+    solver = sim.VIV2D(pc)
+    solver.set_body(v=v0)
+    f = dev(f0)
+    hs = []
+    for n in (1, 14, 25):
+        f, _, hh = solver.run(f, n)
+        hs.append(to_np(hh))
+    solver.check()
+    fr, d, v, a, dh, hhist = S.viv_run(oc, f0, 40, v=v0)
+    assert_close(f, fr, 5e-5, "split f")
+    assert_close(np.concatenate(hs), hhist, 1e-4, "split h")
+    a_, v_, d_ = solver.state()
+    assert np.allclose(d_, d, rtol=1e-4, atol=1e-12)
+    assert np.allclose(v_, v, rtol=1e-4, atol=1e-12)
 
 
 def test_viv_zero_iterations_is_plain_step(sim):

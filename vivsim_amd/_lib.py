@@ -52,7 +52,7 @@ class VsxIB2D(ctypes.Structure):
                 ("mass", c_double), ("stiffness", c_double), ("damping", c_double), ("area", c_double),
                 ("dt", c_double), ("gamma", c_double), ("beta", c_double),
                 ("body", c_void_p), ("g_win", c_void_p), ("hist_d", c_void_p), ("hist_h", c_void_p),
-                ("win_origin", c_void_p)]
+                ("win_origin", c_void_p), ("step0", c_int32), ("fused", c_int32)]
 
 
 _CT = {"p": c_void_p, "i": c_int32, "l": c_int64, "f": c_float, "d": c_double}

@@ -55,15 +55,15 @@ struct Store;
 template <>
 struct Store<VSX_STORE_F32> {
   using T = float;
-  __device__ static inline float ld(const T *p, size_t i, float) { return p[i]; }
-  __device__ static inline void st(T *p, size_t i, float v, float) { p[i] = v; }
+  __device__ static inline float ld(const T *p, uint32_t i, float) { return p[i]; }
+  __device__ static inline void st(T *p, uint32_t i, float v, float) { p[i] = v; }
 };
 
 template <>
 struct Store<VSX_STORE_F16S> {
   using T = _Float16;
-  __device__ static inline float ld(const T *p, size_t i, float w) { return (float)p[i] + w; }
-  __device__ static inline void st(T *p, size_t i, float v, float w) { p[i] = (_Float16)(v - w); }
+  __device__ static inline float ld(const T *p, uint32_t i, float w) { return (float)p[i] + w; }
+  __device__ static inline void st(T *p, uint32_t i, float v, float w) { p[i] = (_Float16)(v - w); }
 };
 
 __device__ inline float bf16_to_f32(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
@@ -77,8 +77,8 @@ __device__ inline uint16_t f32_to_bf16(float v) {
 template <>
 struct Store<VSX_STORE_BF16S> {
   using T = uint16_t;
-  __device__ static inline float ld(const T *p, size_t i, float w) { return bf16_to_f32(p[i]) + w; }
-  __device__ static inline void st(T *p, size_t i, float v, float w) { p[i] = f32_to_bf16(v - w); }
+  __device__ static inline float ld(const T *p, uint32_t i, float w) { return bf16_to_f32(p[i]) + w; }
+  __device__ static inline void st(T *p, uint32_t i, float v, float w) { p[i] = f32_to_bf16(v - w); }
 };
 
 inline size_t store_bytes(int s) { return s == VSX_STORE_F32 ? 4 : 2; }

@@ -52,9 +52,21 @@ struct StepCtx {
   VsxWall at[6];
 };
 
+// The kernel's StepCtx argument, addressed in the kernarg segment (it is always
+// the first kernel parameter, at offset 0).  Taking the context by reference
+// from here, instead of from the by-value parameter, keeps the compiler from
+// making a private per-lane copy of the ~1.4 KB block whenever a reference to
+// it reaches a function that is not inlined.
+__device__ __forceinline__ const StepCtx &kernel_ctx() {
+  return *reinterpret_cast<const StepCtx *>((const char *)__builtin_amdgcn_kernarg_segment_ptr());
+}
+
+// Cell index (< 2^31 cells, checked on the host).  Populations are addressed as
+// (uniform base of population i) + (32-bit cell offset), so every load is a
+// scalar-base + 32-bit vector-offset access.
 template <class L>
-__device__ __forceinline__ int64_t cell_of(const StepCtx &c, int x, int y, int z) {
-  return ((int64_t)x * c.dims[1] + y) * c.dims[2] + z;
+__device__ __forceinline__ uint32_t cell_of(const StepCtx &c, int x, int y, int z) {
+  return ((uint32_t)x * (uint32_t)c.dims[1] + (uint32_t)y) * (uint32_t)c.dims[2] + (uint32_t)z;
 }
 
 __device__ __forceinline__ int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
@@ -67,15 +79,15 @@ __device__ __forceinline__ void pull(const StepCtx &c, const typename SRC::T *__
     const int xs = wrap(x - L::cx[i], c.dims[0]);
     const int ys = wrap(y - L::cy[i], c.dims[1]);
     const int zs = L::D == 3 ? wrap(z - L::cz[i], c.dims[2]) : 0;
-    f[i] = SRC::ld(src, (size_t)i * c.n + cell_of<L>(c, xs, ys, zs), L::w[i]);
+    f[i] = SRC::ld(src + (size_t)i * c.n, cell_of<L>(c, xs, ys, zs), L::w[i]);
   }
 }
 
 template <class L, class SRC>
-__device__ __forceinline__ void local(const StepCtx &c, const typename SRC::T *__restrict__ src, int64_t cell,
+__device__ __forceinline__ void local(const StepCtx &c, const typename SRC::T *__restrict__ src, uint32_t cell,
                                       float (&f)[L::Q]) {
 #pragma unroll
-  for (int i = 0; i < L::Q; ++i) f[i] = SRC::ld(src, (size_t)i * c.n + cell, L::w[i]);
+  for (int i = 0; i < L::Q; ++i) f[i] = SRC::ld(src + (size_t)i * c.n, cell, L::w[i]);
 }
 
 // component a of a coordinate triple, by selects (no dynamic register indexing)
@@ -115,13 +127,16 @@ __device__ __forceinline__ bool on_any_wall(const StepCtx &c, const int (&co)[3]
 }
 
 // Post-streaming, post-walls populations of cell co after walls [0, J).
-template <class L, class SRC, int J>
+// TAG only separates instantiations: an out-of-line function takes its register
+// budget from the kernels that call it, so a kernel with a tight budget calls
+// its own copy.
+template <class L, class SRC, int J, int TAG = 0>
 __device__ __noinline__ void wall_state(const StepCtx &c, const typename SRC::T *__restrict__ src, const int (&co)[3],
                                         float (&f)[L::Q]) {
   if constexpr (J == 0) {
     pull<L, SRC>(c, src, co[0], co[1], co[2], f);
   } else {
-    wall_state<L, SRC, J - 1>(c, src, co, f);
+    wall_state<L, SRC, J - 1, TAG>(c, src, co, f);
     if (J - 1 < c.p.n_walls) {
       const VsxWall &w = c.p.walls[J - 1];
       if (on_wall<L>(c, w.loc, co)) {
@@ -131,7 +146,7 @@ __device__ __noinline__ void wall_state(const StepCtx &c, const typename SRC::T 
         if (wall_needs_neighbor(w)) {
           int cn[3];
           inward<L>(w.loc, co, cn);
-          wall_state<L, SRC, J - 1>(c, src, cn, fn);
+          wall_state<L, SRC, J - 1, TAG>(c, src, cn, fn);
         }
         if (wall_needs_pre(w)) local<L, SRC>(c, src, cell_of<L>(c, co[0], co[1], co[2]), fp);
         apply_wall<L>(w, wall_node<L>(c, w.loc, co), f, fn, fp);
@@ -211,13 +226,13 @@ __device__ __forceinline__ void single_wall_state(const StepCtx &c, const typena
 }
 
 // Post-streaming, post-walls populations of any cell.
-template <class L, class SRC>
+template <class L, class SRC, int TAG = 0>
 __device__ __forceinline__ void cell_state(const StepCtx &c, const typename SRC::T *__restrict__ src, const int (&co)[3],
                                            float (&f)[L::Q]) {
   const int cls = c.p.n_walls > 0 ? wall_class<L>(c, co) : -1;
   if (cls == -1) pull<L, SRC>(c, src, co[0], co[1], co[2], f);
   else if (cls >= 0) single_wall_state<L, SRC>(c, src, co, cls, f);
-  else wall_state<L, SRC, L::n_walls>(c, src, co, f);
+  else wall_state<L, SRC, L::n_walls, TAG>(c, src, co, f);
 }
 
 // Whether any cell can be of class -2: two walls on different axes (shared
@@ -231,31 +246,43 @@ inline bool walls_need_edge(const VsxLbm &p) {
 }
 
 // body force at a cell: uniform + optional full field + immersed-boundary window
+// (window origin `org`, used when `win`)
 template <class L>
-__device__ __forceinline__ void body_force(const StepCtx &c, int64_t cell, const int (&co)[3], float (&g)[3]) {
+__device__ __forceinline__ void body_force(const StepCtx &c, uint32_t cell, const int (&co)[3], const int (&org)[3],
+                                           bool win, float (&g)[3]) {
 #pragma unroll
   for (int a = 0; a < 3; ++a) g[a] = a < L::D ? c.p.g_uniform[a] : 0.f;
   if (c.p.g) {
 #pragma unroll
     for (int a = 0; a < L::D; ++a) g[a] += c.p.g[(size_t)a * c.n + cell];
   }
-  if (c.win.origin) {
+  if (win) {
     int r[3];
     bool in = true;
 #pragma unroll
     for (int a = 0; a < L::D; ++a) {
-      r[a] = co[a] - c.win.origin[a];
+      r[a] = co[a] - org[a];
       in = in && r[a] >= 0 && r[a] < c.win.size[a];
     }
     if (in) {
-      int64_t k = r[0];
+      int k = r[0];
 #pragma unroll
       for (int a = 1; a < L::D; ++a) k = k * c.win.size[a] + r[a];
-      int64_t wn = (int64_t)c.win.size[0] * c.win.size[1] * (L::D == 3 ? c.win.size[2] : 1);
+      const int wn = c.win.size[0] * c.win.size[1] * (L::D == 3 ? c.win.size[2] : 1);
 #pragma unroll
       for (int a = 0; a < L::D; ++a) g[a] += c.win.g[a * wn + k];
     }
   }
+}
+
+// window origin of the bulk kernels: read from the window buffer (device)
+template <class L>
+__device__ __forceinline__ bool window_origin_of(const StepCtx &c, int (&org)[3]) {
+  org[0] = org[1] = org[2] = 0;
+  if (!c.win.origin) return false;
+#pragma unroll
+  for (int a = 0; a < L::D; ++a) org[a] = c.win.origin[a];
+  return true;
 }
 
 // collision + forcing of one cell, composed as in the reference examples:
@@ -314,17 +341,18 @@ __device__ __forceinline__ void collide_force(float (&f)[L::Q], float rho, const
 }
 
 template <class L, class DST>
-__device__ __forceinline__ void store_cell(const StepCtx &c, typename DST::T *__restrict__ dst, int64_t cell,
+__device__ __forceinline__ void store_cell(const StepCtx &c, typename DST::T *__restrict__ dst, uint32_t cell,
                                            const float (&f)[L::Q]) {
 #pragma unroll
-  for (int i = 0; i < L::Q; ++i) DST::st(dst, (size_t)i * c.n + cell, f[i], L::w[i]);
+  for (int i = 0; i < L::Q; ++i) DST::st(dst + (size_t)i * c.n, cell, f[i], L::w[i]);
 }
 
 template <class L, int COLL>
-__device__ __forceinline__ void finish(const StepCtx &c, int64_t cell, const int (&co)[3], float (&f)[L::Q]) {
+__device__ __forceinline__ void finish(const StepCtx &c, uint32_t cell, const int (&co)[3], const int (&org)[3],
+                                       bool win, float (&f)[L::Q]) {
   float rho, u[3], g[3];
   macroscopic<L>(f, rho, u);
-  body_force<L>(c, cell, co, g);
+  body_force<L>(c, cell, co, org, win, g);
   collide_force<L, COLL>(f, rho, u, g, c.p.forcing, c.p.omega, c.p.mrt_rates);
 }
 
@@ -332,8 +360,9 @@ __device__ __forceinline__ void finish(const StepCtx &c, int64_t cell, const int
 // Bulk: one thread per cell; blockIdx.x tiles the contiguous axis, blockIdx.y
 // (and z in 3D) the others, so a wavefront touches 64 consecutive cells.
 template <class L, int COLL, int PHASE, class SRC, class DST>
-__global__ __launch_bounds__(256) void k_bulk(StepCtx c, const typename SRC::T *__restrict__ src,
+__global__ __launch_bounds__(256) void k_bulk(StepCtx c_arg, const typename SRC::T *__restrict__ src,
                                               typename DST::T *__restrict__ dst) {
+  const StepCtx &c = kernel_ctx();
   int co[3];
   if constexpr (L::D == 2) {
     co[1] = blockIdx.x * blockDim.x + threadIdx.x;
@@ -346,7 +375,9 @@ __global__ __launch_bounds__(256) void k_bulk(StepCtx c, const typename SRC::T *
     co[0] = blockIdx.z;
     if (co[2] >= c.dims[2]) return;
   }
-  const int64_t cell = cell_of<L>(c, co[0], co[1], co[2]);
+  const uint32_t cell = cell_of<L>(c, co[0], co[1], co[2]);
+  int org[3];
+  const bool win = window_origin_of<L>(c, org);
   float f[L::Q];
   if constexpr (PHASE == VSX_PHASE_COLLIDE) {
     local<L, SRC>(c, src, cell, f);
@@ -357,21 +388,22 @@ __global__ __launch_bounds__(256) void k_bulk(StepCtx c, const typename SRC::T *
       // register demand out of the interior path below
       if (cls < 0) return;   // corner cells: edge kernel
       single_wall_state<L, SRC>(c, src, co, cls, f);
-      if constexpr (PHASE != VSX_PHASE_STREAM) finish<L, COLL>(c, cell, co, f);
+      if constexpr (PHASE != VSX_PHASE_STREAM) finish<L, COLL>(c, cell, co, org, win, f);
       store_cell<L, DST>(c, dst, cell, f);
       return;
     }
     pull<L, SRC>(c, src, co[0], co[1], co[2], f);
   }
-  if constexpr (PHASE != VSX_PHASE_STREAM) finish<L, COLL>(c, cell, co, f);
+  if constexpr (PHASE != VSX_PHASE_STREAM) finish<L, COLL>(c, cell, co, org, win, f);
   store_cell<L, DST>(c, dst, cell, f);
 }
 
 // Edge: the cells of wall class -2, enumerated as (wall, node) pairs; a cell on
 // several walls is handled under the first wall (in list order) that contains it.
 template <class L, int PHASE, class SRC, class DST>
-__global__ __launch_bounds__(256) void k_edge(StepCtx c, const typename SRC::T *__restrict__ src,
+__global__ __launch_bounds__(256) void k_edge(StepCtx c_arg, const typename SRC::T *__restrict__ src,
                                               typename DST::T *__restrict__ dst, int64_t per_wall) {
+  const StepCtx &c = kernel_ctx();
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int k = (int)(t / per_wall);
   const int64_t j = t % per_wall;
@@ -394,15 +426,17 @@ __global__ __launch_bounds__(256) void k_edge(StepCtx c, const typename SRC::T *
   }
   if (on_any_wall<L>(c, co, k)) return;     // handled under the first wall that contains it
   if (wall_class<L>(c, co) != -2) return;   // bulk kernel
-  const int64_t cell = cell_of<L>(c, co[0], co[1], co[2]);
+  const uint32_t cell = cell_of<L>(c, co[0], co[1], co[2]);
   float f[L::Q];
   wall_state<L, SRC, L::n_walls>(c, src, co, f);
   if constexpr (PHASE == VSX_PHASE_FULL) {
+    int org[3];
+    const bool win = window_origin_of<L>(c, org);
     switch (c.p.collision) {
-      case VSX_COLL_BGK: finish<L, VSX_COLL_BGK>(c, cell, co, f); break;
-      case VSX_COLL_MRT: finish<L, VSX_COLL_MRT>(c, cell, co, f); break;
-      case VSX_COLL_KBC: finish<L, VSX_COLL_KBC>(c, cell, co, f); break;
-      default: finish<L, VSX_COLL_REG>(c, cell, co, f); break;
+      case VSX_COLL_BGK: finish<L, VSX_COLL_BGK>(c, cell, co, org, win, f); break;
+      case VSX_COLL_MRT: finish<L, VSX_COLL_MRT>(c, cell, co, org, win, f); break;
+      case VSX_COLL_KBC: finish<L, VSX_COLL_KBC>(c, cell, co, org, win, f); break;
+      default: finish<L, VSX_COLL_REG>(c, cell, co, org, win, f); break;
     }
   }
   store_cell<L, DST>(c, dst, cell, f);

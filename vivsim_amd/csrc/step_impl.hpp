@@ -10,7 +10,7 @@ int validate_lbm(const VsxLbm *p) {
   if (!p) return fail(VSX_ERR_VALUE, "null parameters");
   if (p->nx < 3 || p->ny < 3 || (L::D == 3 ? p->nz < 3 : p->nz != 1))
     return fail(VSX_ERR_VALUE, "bad grid %d x %d x %d", p->nx, p->ny, p->nz);
-  if ((int64_t)p->nx * p->ny * p->nz * L::Q >= ((int64_t)1 << 40)) return fail(VSX_ERR_VALUE, "grid too large");
+  if ((int64_t)p->nx * p->ny * p->nz >= ((int64_t)1 << 31)) return fail(VSX_ERR_VALUE, "grid too large (>= 2^31 cells)");
   if (p->collision < VSX_COLL_BGK || p->collision > VSX_COLL_REG) return fail(VSX_ERR_VALUE, "unknown collision %d", p->collision);
   if (p->forcing < VSX_FORCE_NONE || p->forcing > VSX_FORCE_EDM) return fail(VSX_ERR_VALUE, "unknown forcing %d", p->forcing);
   if (p->storage < VSX_STORE_F32 || p->storage > VSX_STORE_BF16S) return fail(VSX_ERR_VALUE, "unknown storage %d", p->storage);

@@ -198,7 +198,7 @@ class VIVCase:
 class VIV2D:
     """Device state and fused run of a ``VIVCase``."""
 
-    def __init__(self, case: VIVCase, storage="f32"):
+    def __init__(self, case: VIVCase, storage="f32", fused=True):
         self.case = case
         dev = L.device()
         self.lbm = LBM((case.nx, case.ny), case.omega, collision=case.collision, walls=case.walls(),
@@ -208,8 +208,11 @@ class VIV2D:
         self.marker_y = torch.as_tensor(my, dtype=torch.float64, device=dev)
         S = case.ib_size
         self.g_win = torch.zeros(2 * S * S, dtype=torch.float32, device=dev)
-        self.win_origin = torch.zeros(2, dtype=torch.int32, device=dev)
-        self.body = torch.zeros(6, dtype=torch.float64, device=dev)     # a0 a1 v0 v1 d0 d1
+        # origin x, y of the last step's window, ready flag, spin-timeout flag
+        self.win_origin = torch.zeros(4, dtype=torch.int32, device=dev)
+        # two slots of [a0 a1 v0 v1 d0 d1]: slot (t & 1) holds the state before step t
+        self.body = torch.zeros((2, 6), dtype=torch.float64, device=dev)
+        self.t_total = 0
         ib = L.VsxIB2D()
         ib.n_markers = case.n_markers
         ib.kernel = L.KERNEL[case.kernel]
@@ -225,15 +228,25 @@ class VIV2D:
         ib.body = self.body.data_ptr()
         ib.g_win = self.g_win.data_ptr()
         ib.win_origin = self.win_origin.data_ptr()
+        ib.fused = int(bool(fused))
         self.ib = ib
         self._hist = None
 
     def set_body(self, a=(0.0, 0.0), v=(0.0, 0.0), d=(0.0, 0.0)):
-        self.body.copy_(torch.tensor(list(a) + list(v) + list(d), dtype=torch.float64))
+        """Reset the structure state (and the step counter of the body slots)."""
+        self.body.zero_()
+        self.body[0].copy_(torch.tensor(list(a) + list(v) + list(d), dtype=torch.float64))
+        self.win_origin.zero_()
+        self.t_total = 0
 
     def state(self):
-        b = self.body.cpu().numpy()
+        b = self.body[self.t_total & 1].cpu().numpy()
         return b[0:2].copy(), b[2:4].copy(), b[4:6].copy()   # a, v, d
+
+    def check(self):
+        """Raise if a force-window tile of a fused step timed out waiting for the IB workgroup."""
+        if int(self.win_origin[3].item()) != 0:
+            raise RuntimeError("fused VIV step: window tile timed out waiting for the IB workgroup")
 
     def run(self, f, n_steps, out=None, history=True):
         """Advance n_steps; returns (f_out, d_hist [n,2], h_hist [n,2]) (histories FP64 on device)."""
@@ -246,6 +259,8 @@ class VIV2D:
         self.ib.hist_d = None if hd is None else hd.data_ptr()
         self.ib.hist_h = None if hh is None else hh.data_ptr()
         wa, wb = self.lbm.work()
+        self.ib.step0 = self.t_total
         L.call("vsx_viv2d_run", ctypes.byref(self.lbm.p), ctypes.byref(self.ib), L.ptr(f), L.ptr(out), L.ptr(wa),
                L.ptr(wb), int(n_steps))
+        self.t_total += int(n_steps)
         return out, hd, hh
