@@ -38,12 +38,12 @@ def kernel_times(lib, n):
     return np.array(ms[: got.value], dtype=np.float64)
 
 
-def run_gpu(args, storage, steps, warmup):
+def run_gpu(args, storage, steps, warmup, fused=True):
     import torch
     from vivsim_amd import _lib, lbm, sim
 
     case = sim.VIVCase(nx=args.nx, ny=args.ny)
-    solver = sim.VIV2D(case, storage=storage)
+    solver = sim.VIV2D(case, storage=storage, fused=fused)
     solver.set_body(v=(0.0, 1e-3 * case.U0))
     u = torch.as_tensor(case.initial_u(), dtype=torch.float32, device="cuda")
     f = lbm.get_equilibrium(torch.ones(case.nx, case.ny, device="cuda"), u)
@@ -57,6 +57,7 @@ def run_gpu(args, storage, steps, warmup):
     t1 = time.perf_counter()
     kt = kernel_times(lib, steps)
     assert torch.isfinite(out).all().item(), "non-finite state"
+    solver.check()
     cells = case.nx * case.ny
     sec = t1 - t0
     bulk_cells = case.nx * case.ny                  # every cell, wall columns included, is in the fused launch
@@ -123,6 +124,10 @@ def main():
                 r = run_gpu(args, st, args.steps, args.warmup)
                 variants[st] = dict(glups=round(r["glups"], 3), ms_per_step=round(r["ms_per_step"], 5),
                                     kernel_us=round(r["kernel_ms"] * 1e3, 3), achieved_gbs=round(r["achieved_gbs"], 1))
+        # the same FP32 step as two launches (IB kernel, then the bulk kernel)
+        r = run_gpu(args, args.storage, args.steps, args.warmup, fused=False)
+        variants["separate_launches"] = dict(glups=round(r["glups"], 3), ms_per_step=round(r["ms_per_step"], 5),
+                                             bulk_kernel_us=round(r["kernel_ms"] * 1e3, 3))
     cpu = None if args.no_cpu else run_cpu(args, args.cpu_seconds)
 
     line = {

@@ -111,6 +111,28 @@ typedef struct VsxIB2D {
                           pass; 0: separate IB and bulk kernels                              */
 } VsxIB2D;
 
+/* 3D immersed body held fixed in a D3Q19 VsxLbm step (flow past a sphere).
+ * Step composition: examples/3d/flow_past_sphere.py:139-171; the stencil of the
+ * fixed markers is computed once, relative to the window origin, as in
+ * flow_past_sphere.py:114-121 (vsx_ib_get_ib_stencil3d on window-relative
+ * coordinates), and transposed once with vsx_ib3d_transpose_stencil. */
+typedef struct VsxIB3D {
+  int32_t n_markers, n_iter;
+  int32_t origin[3];          /* window origin in the grid (window inside the grid)            */
+  int32_t win[3];             /* window sizes (S0, S1, S2); cells = S0 * S1 * S2               */
+  const float *W;             /* device [M][64] stencil weights                                */
+  const int32_t *I;           /* device [M][64] flat window indices                            */
+  const int32_t *cell_ptr;    /* device [cells + 1]: stencil slots of window cell k are         */
+  const int32_t *cell_slot;   /*   cell_slot[cell_ptr[k] .. cell_ptr[k+1]) (slot = m * 64 + s) */
+  const float *u_target;      /* device [M][3] target marker velocity                          */
+  const float *ds_arr;        /* device [M] or NULL -> ds                                      */
+  float ds;
+  int32_t reserved;
+  float *g_win;               /* device [3][cells]: force window of the last step              */
+  float *work;                /* device scratch, vsx_ib3d_work_floats(M, cells) floats           */
+  double *hist_h;             /* optional device [n_steps][3]: total marker force (FP64)        */
+} VsxIB3D;
+
 /* ------------------------------------------------------------------ library */
 const char *vsx_version(void);
 const char *vsx_last_error(void);
@@ -207,6 +229,22 @@ int vsx_d3q19_run(const VsxLbm *p, const float *f_in, float *f_out, void *work_a
  * force (FP64) -> Newmark (FP64) -> collision + forcing -> streaming -> walls. */
 int vsx_viv2d_run(const VsxLbm *p, const VsxIB2D *ib, const float *f_in, float *f_out,
                   void *work_a, void *work_b, int32_t n_steps, void *stream);
+
+/* n_steps of the 3D IB-LBM step past a fixed body (examples/3d/flow_past_sphere.py:139-171):
+ * per step: macroscopic -> IB window -> multi-direct forcing (ib/mdf.py:10-64 with the
+ * precomputed stencil) -> total marker force (FP64) -> collision + forcing -> streaming
+ * -> walls.  p must be a D3Q19 step. */
+int vsx_sphere3d_run(const VsxLbm *p, const VsxIB3D *ib, const float *f_in, float *f_out,
+                     void *work_a, void *work_b, int32_t n_steps, void *stream);
+/* Size in floats of VsxIB3D.work. */
+int64_t vsx_ib3d_work_floats(int32_t n_markers, int64_t cells);
+/* Transpose of a stencil (I [m][s] flat indices into a grid of n_grid cells):
+ * cell_ptr [n_grid + 1], cell_slot [m * s] list, for each cell in ascending
+ * order, the slots m * s_len + j whose index lands on it under the scatter
+ * rule of ib/stencil.py:106-110 (out-of-range entries dropped).  tmp: device
+ * scratch of n_grid + 1 int32. */
+int vsx_ib3d_transpose_stencil(const int32_t *I, int32_t m, int32_t s, int64_t n_grid, int32_t *cell_ptr,
+                               int32_t *cell_slot, int32_t *tmp, void *stream);
 
 /* ------------------------------------------------------------------ immersed boundary */
 

@@ -22,12 +22,12 @@ def test_library_exports_header_symbols():
     assert not missing, missing
     assert lib.vsx_version().startswith(b"vivsim_amd")
     # every declared entry point has a ctypes signature
-    assert set(syms) - set(L._SIGS) == {"vsx_version", "vsx_last_error"}
+    assert set(syms) - set(L._SIGS) == L.NON_STATUS
 
 
 def test_struct_layout_matches_c(tmp_path):
     src = tmp_path / "layout.c"
-    fields = {"VsxWall": L.VsxWall, "VsxLbm": L.VsxLbm, "VsxIB2D": L.VsxIB2D}
+    fields = {"VsxWall": L.VsxWall, "VsxLbm": L.VsxLbm, "VsxIB2D": L.VsxIB2D, "VsxIB3D": L.VsxIB3D}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "vivsim_amd.h"', "int main(void) {"]
     for sname, cls in fields.items():
         lines.append(f'printf("{sname} %zu\\n", sizeof({sname}));')

@@ -176,3 +176,52 @@ def test_fused3d_half(sim, storage, tol):
     got = sim.LBM(sh, om, collision="bgk", walls=walls, storage=storage).run(dev(f0), n)
     ref = run_oracle(f0, n, om, walls, "bgk", None, None)
     assert_close(got, ref, tol, storage)
+
+
+# ------------------------------------------------------------------ sphere IB-LBM (3D)
+
+def small_sphere(cls, **kw):
+    base = dict(nx=40, ny=28, nz=26, D=8.0, cx=14.0, cy=14.0, cz=13.0, n_markers=256, U0=0.05, Re=40.0, n_iter=2,
+                pad=3)
+    base.update(kw)
+    return cls(**base)
+
+
+@pytest.mark.parametrize("variant", [dict(), dict(n_iter=3, collision="mrt", forcing="guo"),
+                                     dict(n_iter=1, inlet="nebb", outlet="eq"), dict(n_iter=0)],
+                         ids=["bgk_edm_2it", "mrt_guo_3it", "nebb_1it", "no_ib"])
+def test_sphere_matches_oracle(sim, variant):
+    oc, pc = small_sphere(S.Sphere3D, **variant), small_sphere(sim.SphereCase, **variant)
+    steps = 12
+    say(f"sphere {variant}")
+    solver = sim.Sphere3D(pc)
+    f0 = oc.initial_f(np.float64)
+    got, hh = solver.run(dev(f0), steps)
+    W, I = oc.stencil(np.float32)
+    assert np.array_equal(to_np(solver.I), I), "stencil indices"
+    f = f0.astype(np.float64)
+    hs = []
+    for _ in range(steps):
+        f, h = S.sphere_step(oc, f, W.astype(np.float64), I)
+        hs.append(h)
+    assert_close(got, f, 5e-5, "sphere f")
+    if oc.n_iter > 0:
+        assert_close(hh, np.array(hs), 1e-4, "sphere force history")
+    else:
+        assert torch.all(hh == 0)
+
+
+def test_sphere_transpose_is_exact(sim):
+    """The transposed stencil lists every in-range slot once, per cell in ascending order."""
+    pc = small_sphere(sim.SphereCase)
+    solver = sim.Sphere3D(pc)
+    I = to_np(solver.I).ravel()
+    cells = pc.ib_size ** 3
+    ptr, slot = to_np(solver.cell_ptr), to_np(solver.cell_slot)
+    valid = np.nonzero((I >= 0) & (I < cells))[0]
+    assert ptr[0] == 0 and ptr[-1] == valid.size
+    for k in range(cells):
+        lst = slot[ptr[k]:ptr[k + 1]]
+        assert np.all(np.diff(lst) > 0)
+        assert np.all(I[lst] == k)
+    assert np.array_equal(np.sort(slot[:ptr[-1]]), valid)

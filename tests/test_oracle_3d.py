@@ -186,3 +186,52 @@ def test_characteristic_3d():
         assert np.allclose(r, 1.02) and np.allclose(uu[0], 0.01) and np.allclose(uu[2], 0.03)
     with pytest.raises(ValueError):
         L.boundary_characteristic(rho, u, "north")
+
+
+# ------------------------------------------------------------------ sphere step composition
+
+def _small_sphere(**kw):
+    from oracle import steps as S
+    base = dict(nx=30, ny=24, nz=22, D=8.0, cx=12.0, cy=12.0, cz=11.0, n_markers=200, U0=0.05, Re=40.0, n_iter=2,
+                pad=3)
+    base.update(kw)
+    return S.Sphere3D(**base)
+
+
+def test_sphere_window_equals_full_grid_ib():
+    """IB on the window with relative coordinates == IB on the whole grid with
+    absolute coordinates (the window contains every stencil point)."""
+    from oracle import ib as IB
+    c = _small_sphere()
+    rng = np.random.default_rng(7)
+    u = 0.05 * rng.standard_normal((3, c.nx, c.ny, c.nz))
+    W, I = c.stencil(np.float64)
+    S_ = c.ib_size
+    x0, y0, z0 = c.ib_origin
+    tgt = np.zeros((c.n_markers, 3))
+    gw, hw = IB.multi_direct_forcing(u[:, x0:x0 + S_, y0:y0 + S_, z0:z0 + S_], W, I, tgt, c.ds, c.n_iter)
+    Wf, If = IB.get_ib_stencil3d(c.markers(), (c.nx, c.ny, c.nz), IB.KERNELS[c.kernel], 2)
+    gf, hf = IB.multi_direct_forcing(u, Wf, If, tgt, c.ds, c.n_iter)
+    assert np.allclose(hw, hf, rtol=1e-12, atol=1e-16)
+    assert np.allclose(gw, gf[:, x0:x0 + S_, y0:y0 + S_, z0:z0 + S_], rtol=1e-12, atol=1e-16)
+    assert np.allclose(gw.sum(axis=(1, 2, 3)), -hw.sum(axis=0), rtol=1e-10, atol=1e-15)
+
+
+def test_sphere_zero_iterations_is_plain_step():
+    from oracle import steps as S
+    c = _small_sphere(n_iter=0)
+    f = c.initial_f(np.float64)
+    f1, h = S.sphere_step(c, f)
+    f2 = S.lbm_step3d(f, c.omega, c.walls(), c.collision, c.forcing, None)
+    assert np.allclose(f1, f2, rtol=0, atol=1e-15) and np.all(h == 0)
+
+
+def test_sphere_drag_points_downstream():
+    """After a few steps of uniform inflow the fluid force on the markers
+    opposes the flow (h = -F has the sign of the drag on the body, +x)."""
+    from oracle import steps as S
+    c = _small_sphere()
+    f = c.initial_f(np.float64)
+    for _ in range(3):
+        f, h = S.sphere_step(c, f)
+    assert h[0] > 0 and abs(h[1]) < 1e-2 * h[0] and abs(h[2]) < 1e-2 * h[0]

@@ -55,6 +55,13 @@ class VsxIB2D(ctypes.Structure):
                 ("win_origin", c_void_p), ("step0", c_int32), ("fused", c_int32)]
 
 
+class VsxIB3D(ctypes.Structure):
+    _fields_ = [("n_markers", c_int32), ("n_iter", c_int32), ("origin", c_int32 * 3), ("win", c_int32 * 3),
+                ("W", c_void_p), ("I", c_void_p), ("cell_ptr", c_void_p), ("cell_slot", c_void_p),
+                ("u_target", c_void_p), ("ds_arr", c_void_p), ("ds", c_float), ("reserved", c_int32),
+                ("g_win", c_void_p), ("work", c_void_p), ("hist_h", c_void_p)]
+
+
 _CT = {"p": c_void_p, "i": c_int32, "l": c_int64, "f": c_float, "d": c_double}
 
 # name -> argument codes (all return int except the two string getters)
@@ -81,6 +88,8 @@ _SIGS = {
     "vsx_dyn_markers_coords": "ppiddpipp p",
     "vsx_dyn_markers_velocity_3dof": "ppiddppp p",
     "vsx_viv2d_run": "pppppp i p",
+    "vsx_sphere3d_run": "pppppp i p",
+    "vsx_ib3d_transpose_stencil": "piilppp p",
 }
 for _pfx in ("d2q9", "d3q19"):
     _SIGS.update({
@@ -107,6 +116,9 @@ _SIGS.update({
     "vsx_d3q19_boundary_characteristic": "ppippiii p",
 })
 
+# declared in _declare with their own return types
+NON_STATUS = {"vsx_version", "vsx_last_error", "vsx_ib3d_work_floats"}
+
 _LIB = None
 
 
@@ -115,6 +127,9 @@ def _declare(L):
         fn = getattr(L, name)
         fn.restype = c_int32
         fn.argtypes = [_CT[c] for c in codes.replace(" ", "")]
+    # entry points that do not return a status code
+    L.vsx_ib3d_work_floats.restype = c_int64
+    L.vsx_ib3d_work_floats.argtypes = [c_int32, c_int64]
     for name in ("vsx_version", "vsx_last_error"):
         fn = getattr(L, name)
         fn.restype = ctypes.c_char_p

@@ -1,0 +1,341 @@
+// 3D IB-LBM step past a fixed body (flow past a sphere), D3Q19.
+//
+// Per step (examples/3d/flow_past_sphere.py:139-171, ib/mdf.py:10-64):
+//   k_win_u3d      window velocity of the post-streaming state          (cells)
+//   k_mark3d(0)    ub = interp(u); dF = (U - ub) 2 ds; F = dF             (markers)
+//   k_spread3d     A = spread(dF)                                        (cells)
+//   k_mark3d(k)    ub += interp(0.5 A); dF = ...; F += dF                (markers)
+//   ...            (n_iter - 1 times)
+//   k_spread3d     g_win = spread(F); total marker force -F in FP64      (cells)
+//   bulk           collision + forcing with g_win, streaming, walls      (cells)
+// The body is fixed, so its stencil (computed once, relative to the window
+// origin) is transposed once: spread becomes a gather over the markers that
+// touch each window cell, in ascending marker order -- deterministic and free
+// of atomics.  The last spread of the reference loop (its interpolation only
+// updates ub, which is not used afterwards) is not evaluated.
+#include "ib_common.hpp"
+#include "step_impl.hpp"
+
+namespace vsx {
+
+constexpr int kS3 = 64;   // stencil points per marker (4 x 4 x 4)
+
+struct IB3DDev {
+  int32_t *origin;   // [4] window origin (read by the bulk kernel)
+  float *u_win, *acc, *ub, *F, *dF, *g_win;
+  const float *W;
+  const int32_t *I, *cell_ptr, *cell_slot;
+  const float *u_target, *ds_arr;
+  float ds;
+  int32_t M, n_iter;
+  int32_t S[3];
+  int32_t org[3];
+  int64_t cells;
+};
+
+int64_t work_floats(int32_t m, int64_t cells) { return 4 + 2 * 3 * cells + 3 * 3 * (int64_t)m; }
+
+IB3DDev make_dev(const VsxIB3D &ib) {
+  IB3DDev d;
+  d.cells = (int64_t)ib.win[0] * ib.win[1] * ib.win[2];
+  float *w = ib.work;
+  d.origin = reinterpret_cast<int32_t *>(w);
+  d.u_win = w + 4;
+  d.acc = d.u_win + 3 * d.cells;
+  d.ub = d.acc + 3 * d.cells;
+  d.F = d.ub + 3 * (int64_t)ib.n_markers;
+  d.dF = d.F + 3 * (int64_t)ib.n_markers;
+  d.g_win = ib.g_win;
+  d.W = ib.W;
+  d.I = ib.I;
+  d.cell_ptr = ib.cell_ptr;
+  d.cell_slot = ib.cell_slot;
+  d.u_target = ib.u_target;
+  d.ds_arr = ib.ds_arr;
+  d.ds = ib.ds;
+  d.M = ib.n_markers;
+  d.n_iter = ib.n_iter;
+  for (int a = 0; a < 3; ++a) {
+    d.S[a] = ib.win[a];
+    d.org[a] = ib.origin[a];
+  }
+  return d;
+}
+
+// Window velocity (3 components) of the post-streaming state: pulled from the
+// post-collision state with the walls applied (PULL), or read in place (the
+// first step of a run reads the caller's post-streaming state).
+template <class SRC, bool PULL>
+__global__ __launch_bounds__(256) void k_win_u3d(StepCtx c_arg, IB3DDev d, const typename SRC::T *__restrict__ src) {
+  const StepCtx &c = kernel_ctx();
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k == 0) {
+    for (int a = 0; a < 3; ++a) d.origin[a] = d.org[a];
+  }
+  if (k >= d.cells) return;
+  const int r2 = (int)(k % d.S[2]);
+  const int r1 = (int)((k / d.S[2]) % d.S[1]);
+  const int r0 = (int)(k / ((int64_t)d.S[2] * d.S[1]));
+  const int co[3] = {d.org[0] + r0, d.org[1] + r1, d.org[2] + r2};
+  float f[19], rho, u[3];
+  if constexpr (PULL) cell_state<D3Q19, SRC>(c, src, co, f);
+  else local<D3Q19, SRC>(c, src, cell_of<D3Q19>(c, co[0], co[1], co[2]), f);
+  macroscopic<D3Q19>(f, rho, u);
+  for (int a = 0; a < 3; ++a) d.u_win[a * d.cells + k] = u[a];
+}
+
+// Marker phase `it` of the multi-direct forcing loop (ib/mdf.py:45-57).
+// it = 0: ub = interp(u_win), F = 0; it > 0: ub += interp(0.5 * acc).
+// Then, when it < n_iter: dF = (U - ub) * 2 ds, F += dF.
+__global__ __launch_bounds__(256) void k_mark3d(IB3DDev d, int it) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= d.M) return;
+  const float *grid = it == 0 ? d.u_win : d.acc;
+  const float *W = d.W + (int64_t)m * kS3;
+  const int32_t *I = d.I + (int64_t)m * kS3;
+  float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+  for (int j = 0; j < kS3; ++j) {
+    const int64_t gi = gather_index(I[j], d.cells);
+    const float w = W[j];
+    if (it == 0) {
+      s0 += w * grid[gi];
+      s1 += w * grid[d.cells + gi];
+      s2 += w * grid[2 * d.cells + gi];
+    } else {
+      s0 += w * (grid[gi] * 0.5f);
+      s1 += w * (grid[d.cells + gi] * 0.5f);
+      s2 += w * (grid[2 * d.cells + gi] * 0.5f);
+    }
+  }
+  float ub[3], F[3];
+  if (it == 0) {
+    ub[0] = s0, ub[1] = s1, ub[2] = s2;
+    F[0] = F[1] = F[2] = 0.f;
+  } else {
+    ub[0] = d.ub[3 * m] + s0, ub[1] = d.ub[3 * m + 1] + s1, ub[2] = d.ub[3 * m + 2] + s2;
+    F[0] = d.F[3 * m], F[1] = d.F[3 * m + 1], F[2] = d.F[3 * m + 2];
+  }
+  if (it < d.n_iter) {
+    const float ds2 = (d.ds_arr ? d.ds_arr[m] : d.ds) * 2.0f;
+    for (int a = 0; a < 3; ++a) {
+      const float df = (d.u_target[3 * m + a] - ub[a]) * ds2;
+      F[a] += df;
+      d.dF[3 * m + a] = df;
+    }
+  }
+  for (int a = 0; a < 3; ++a) {
+    d.ub[3 * m + a] = ub[a];
+    d.F[3 * m + a] = F[a];
+  }
+}
+
+__device__ __forceinline__ double wave_sum3(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// out[a][k] = sum over the stencil slots that land on window cell k of
+// mf[m][a] * W[slot] (ib/stencil.py:81-110 as a gather).  With hist, block 0
+// also writes the total marker force sum_m (-F_m) in FP64 (dyn.py:126-136).
+__global__ __launch_bounds__(256) void k_spread3d(IB3DDev d, const float *__restrict__ mf, float *__restrict__ out,
+                                                  double *hist) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k < d.cells) {
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+    const int e1 = d.cell_ptr[k + 1];
+    for (int e = d.cell_ptr[k]; e < e1; ++e) {
+      const int slot = d.cell_slot[e];
+      const int m = slot / kS3;
+      const float w = d.W[slot];
+      s0 += mf[3 * m] * w;
+      s1 += mf[3 * m + 1] * w;
+      s2 += mf[3 * m + 2] * w;
+    }
+    out[k] = s0;
+    out[d.cells + k] = s1;
+    out[2 * d.cells + k] = s2;
+  }
+  if (hist && blockIdx.x == 0) {
+    __shared__ double red[3][4];
+    double h[3] = {0.0, 0.0, 0.0};
+    for (int m = threadIdx.x; m < d.M; m += blockDim.x)
+      for (int a = 0; a < 3; ++a) h[a] -= (double)d.F[3 * m + a];
+    for (int a = 0; a < 3; ++a) h[a] = wave_sum3(h[a]);
+    if ((threadIdx.x & 63) == 0)
+      for (int a = 0; a < 3; ++a) red[a][threadIdx.x >> 6] = h[a];
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int a = 0; a < 3; ++a) hist[a] = red[a][0] + red[a][1] + red[a][2] + red[a][3];
+  }
+}
+
+// ------------------------------------------------------------------ stencil transpose
+
+__global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+__global__ void k_count(const int32_t *I, int64_t n_slots, int64_t n_grid, int32_t *cnt) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n_slots; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t k = scatter_index(I[i], n_grid);
+    if (k >= 0) atomicAdd(&cnt[k], 1);
+  }
+}
+
+// exclusive scan of cnt[0..n) into ptr[0..n] by one workgroup of 1024 threads
+// (each thread scans a contiguous chunk; chunk totals are scanned in LDS)
+__global__ __launch_bounds__(1024) void k_scan(const int32_t *cnt, int64_t n, int32_t *ptr) {
+  __shared__ int32_t tot[1024];
+  const int t = threadIdx.x;
+  const int64_t chunk = (n + 1023) / 1024;
+  const int64_t b = t * chunk, e = b + chunk < n ? b + chunk : n;
+  int32_t s = 0;
+  for (int64_t i = b; i < e; ++i) s += cnt[i];
+  tot[t] = s;
+  __syncthreads();
+  if (t == 0) {
+    int32_t run = 0;
+    for (int j = 0; j < 1024; ++j) {
+      const int32_t v = tot[j];
+      tot[j] = run;
+      run += v;
+    }
+  }
+  __syncthreads();
+  int32_t run = tot[t];
+  for (int64_t i = b; i < e; ++i) {
+    ptr[i] = run;
+    run += cnt[i];
+  }
+  if (t == 1023) ptr[n] = run;
+}
+
+__global__ void k_place(const int32_t *I, int64_t n_slots, int64_t n_grid, const int32_t *ptr, int32_t *cursor,
+                        int32_t *slot_out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n_slots; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t k = scatter_index(I[i], n_grid);
+    if (k >= 0) slot_out[ptr[k] + atomicAdd(&cursor[k], 1)] = (int32_t)i;
+  }
+}
+
+// each cell's list in ascending slot order (insertion sort: lists are short)
+__global__ void k_sort_lists(const int32_t *ptr, int64_t n_grid, int32_t *slot) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n_grid; k += (int64_t)gridDim.x * blockDim.x) {
+    const int b = ptr[k], e = ptr[k + 1];
+    for (int i = b + 1; i < e; ++i) {
+      const int32_t v = slot[i];
+      int j = i - 1;
+      while (j >= b && slot[j] > v) {
+        slot[j + 1] = slot[j];
+        --j;
+      }
+      slot[j + 1] = v;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ host side
+
+template <class SRC, bool PULL>
+void launch_win_u(const StepCtx &c, const IB3DDev &d, const void *src, hipStream_t s) {
+  hipLaunchKernelGGL((k_win_u3d<SRC, PULL>), dim3((unsigned)((d.cells + 255) / 256)), dim3(256), 0, s, c, d,
+                     static_cast<const typename SRC::T *>(src));
+}
+
+void win_u_any(const StepCtx &c, const IB3DDev &d, const void *src, bool first, hipStream_t s) {
+  if (first) return launch_win_u<Store<VSX_STORE_F32>, false>(c, d, src, s);
+  switch (c.p.storage) {
+    case VSX_STORE_F32: return launch_win_u<Store<VSX_STORE_F32>, true>(c, d, src, s);
+    case VSX_STORE_F16S: return launch_win_u<Store<VSX_STORE_F16S>, true>(c, d, src, s);
+    default: return launch_win_u<Store<VSX_STORE_BF16S>, true>(c, d, src, s);
+  }
+}
+
+int ib3d_phase(const StepCtx &c, const IB3DDev &d, const void *src, bool first, double *hist, hipStream_t s) {
+  win_u_any(c, d, src, first, s);
+  const unsigned gm = (unsigned)((d.M + 255) / 256), gc = (unsigned)((d.cells + 255) / 256);
+  hipLaunchKernelGGL(k_mark3d, dim3(gm), dim3(256), 0, s, d, 0);
+  for (int it = 1; it < d.n_iter; ++it) {
+    hipLaunchKernelGGL(k_spread3d, dim3(gc), dim3(256), 0, s, d, (const float *)d.dF, d.acc, (double *)nullptr);
+    hipLaunchKernelGGL(k_mark3d, dim3(gm), dim3(256), 0, s, d, it);
+  }
+  hipLaunchKernelGGL(k_spread3d, dim3(gc), dim3(256), 0, s, d, (const float *)d.F, d.g_win, hist);
+  VSX_CHECK_LAUNCH();
+  return VSX_OK;
+}
+
+int validate_ib3d(const VsxLbm *p, const VsxIB3D *ib) {
+  if (!ib) return fail(VSX_ERR_VALUE, "null IB parameters");
+  VSX_REQUIRE(ib->n_markers > 0, "marker count %d out of range", ib->n_markers);
+  VSX_REQUIRE(ib->n_iter >= 0, "negative n_iter");
+  const int dims[3] = {p->nx, p->ny, p->nz};
+  for (int a = 0; a < 3; ++a)
+    VSX_REQUIRE(ib->win[a] >= 1 && ib->origin[a] >= 0 && ib->origin[a] + ib->win[a] <= dims[a],
+                "IB window axis %d (origin %d, size %d) not inside the grid (%d)", a, ib->origin[a], ib->win[a], dims[a]);
+  VSX_REQUIRE(ib->W && ib->I && ib->cell_ptr && ib->cell_slot && ib->u_target && ib->g_win && ib->work,
+              "null IB buffer");
+  VSX_REQUIRE(p->forcing != VSX_FORCE_NONE, "the IB step needs a forcing scheme (guo or edm)");
+  return VSX_OK;
+}
+
+int sphere_run(const VsxLbm *p, const VsxIB3D *ib, const float *f_in, float *f_out, void *wa, void *wb, int n,
+               void *stream) {
+  int rc = validate_lbm<D3Q19>(p);
+  if (rc) return rc;
+  rc = validate_ib3d(p, ib);
+  if (rc) return rc;
+  VSX_REQUIRE(f_in && f_out && wa, "null buffer");
+  VSX_REQUIRE(n >= 0, "negative step count");
+  VSX_REQUIRE(n <= 1 || wb, "need two work buffers for more than one step");
+  hipStream_t s = as_stream(stream);
+  const IB3DDev d = make_dev(*ib);
+  const StepCtx c = make_ctx<D3Q19>(*p, Window{d.origin, d.g_win, {ib->win[0], ib->win[1], ib->win[2]}});
+  if (n == 0) {
+    if (f_out != f_in) VSX_HIP(hipMemcpyAsync(f_out, f_in, sizeof(float) * 19 * c.n, hipMemcpyDeviceToDevice, s));
+    return VSX_OK;
+  }
+  const void *cur = f_in;
+  void *bufs[2] = {wa, wb};
+  for (int t = 0; t < n; ++t) {
+    void *dst = bufs[t & 1];
+    rc = ib3d_phase(c, d, cur, t == 0, ib->hist_h ? ib->hist_h + 3 * (int64_t)t : nullptr, s);
+    if (rc) return rc;
+    rc = step_any<D3Q19>(c, cur, t == 0, dst, false, t == 0 ? VSX_PHASE_COLLIDE : VSX_PHASE_FULL, s);
+    if (rc) return rc;
+    cur = dst;
+  }
+  const StepCtx cs = make_ctx<D3Q19>(*p, Window{nullptr, nullptr, {0, 0, 0}});
+  return step_any<D3Q19>(cs, cur, false, f_out, true, VSX_PHASE_STREAM, s);
+}
+
+}  // namespace vsx
+
+using namespace vsx;
+
+extern "C" int64_t vsx_ib3d_work_floats(int32_t n_markers, int64_t cells) { return work_floats(n_markers, cells); }
+
+extern "C" int vsx_sphere3d_run(const VsxLbm *p, const VsxIB3D *ib, const float *f_in, float *f_out, void *work_a,
+                                void *work_b, int32_t n_steps, void *stream) {
+  return sphere_run(p, ib, f_in, f_out, work_a, work_b, n_steps, stream);
+}
+
+extern "C" int vsx_ib3d_transpose_stencil(const int32_t *I, int32_t m, int32_t s, int64_t n_grid, int32_t *cell_ptr,
+                                          int32_t *cell_slot, int32_t *tmp, void *stream) {
+  VSX_REQUIRE(I && cell_ptr && cell_slot && tmp, "null buffer");
+  VSX_REQUIRE(m > 0 && s > 0 && n_grid > 0 && n_grid < ((int64_t)1 << 31), "bad sizes m=%d s=%d n_grid=%lld", m, s,
+              (long long)n_grid);
+  VSX_REQUIRE((int64_t)m * s < ((int64_t)1 << 31), "too many stencil slots");
+  hipStream_t st = as_stream(stream);
+  const int64_t slots = (int64_t)m * s;
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(n_grid + 1, 256)), dim3(256), 0, st, tmp, n_grid + 1, 0);
+  hipLaunchKernelGGL(k_count, dim3(grid_for(slots, 256)), dim3(256), 0, st, I, slots, n_grid, tmp);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, (const int32_t *)tmp, n_grid, cell_ptr);
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(n_grid + 1, 256)), dim3(256), 0, st, tmp, n_grid + 1, 0);
+  hipLaunchKernelGGL(k_place, dim3(grid_for(slots, 256)), dim3(256), 0, st, I, slots, n_grid,
+                     (const int32_t *)cell_ptr, tmp, cell_slot);
+  hipLaunchKernelGGL(k_sort_lists, dim3(grid_for(n_grid, 256)), dim3(256), 0, st, (const int32_t *)cell_ptr, n_grid,
+                     cell_slot);
+  VSX_CHECK_LAUNCH();
+  return VSX_OK;
+}

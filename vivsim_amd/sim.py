@@ -9,6 +9,7 @@ same compositions run as fused HIP passes over device-resident buffers:
   ordered walls, for D2Q9 or D3Q19 (``vsx_d2q9_run`` / ``vsx_d3q19_run``).
 * ``VIV2D`` — the 2D IB-LBM vortex-induced-vibration step with multi-direct
   forcing, FP64 body-force reduction and FP64 Newmark (``vsx_viv2d_run``).
+* ``Sphere3D`` — the 3D IB-LBM step past a fixed body (``vsx_sphere3d_run``).
 
 Wall lists use the dictionaries ``{"kind": "nee"|"nebb"|"eq"|"bb"|"specular",
 "mode": "plain"|"velocity"|"pressure"|"force_corrected", "loc": ...,
@@ -264,3 +265,135 @@ class VIV2D:
                L.ptr(wb), int(n_steps))
         self.t_total += int(n_steps)
         return out, hd, hh
+
+
+# ====================================================================== sphere 3D
+
+@dataclass
+class SphereCase:
+    """Flow past a fixed sphere (examples/3d/flow_past_sphere.py:29-124 for the
+    geometry and window definitions): a D3Q19 step with an IB window of side
+    D + 2 pad around the body; markers on a Fibonacci lattice with uniform
+    ds = pi D^2 / N."""
+    nx: int = 256
+    ny: int = 128
+    nz: int = 128
+    D: float = 24.0
+    cx: float = 64.0
+    cy: float = 64.0
+    cz: float = 64.0
+    n_markers: int = 4096
+    U0: float = 0.05
+    Re: float = 200.0
+    n_iter: int = 2
+    pad: int = 4
+    collision: str = "bgk"
+    forcing: str = "edm"
+    kernel: str = "peskin_4pt"
+    inlet: str = "eq"
+    outlet: str = "pressure_nee"
+
+    @property
+    def nu(self):
+        return self.U0 * self.D / self.Re
+
+    @property
+    def omega(self):
+        return 1 / (3 * self.nu + 0.5)
+
+    @property
+    def ib_origin(self):
+        h = 0.5 * self.D + self.pad
+        return int(self.cx - h), int(self.cy - h), int(self.cz - h)
+
+    @property
+    def ib_size(self):
+        return int(self.D + 2 * self.pad)
+
+    @property
+    def ds(self):
+        return math.pi * self.D ** 2 / self.n_markers
+
+    def markers(self):
+        """Fibonacci lattice: z_k = 1 - (2k+1)/N, phi_k = k pi (3 - sqrt 5) (float64, host)."""
+        k = np.arange(self.n_markers)
+        z = 1 - (2 * k + 1) / self.n_markers
+        r = np.sqrt(1 - z * z)
+        ph = k * np.pi * (3 - np.sqrt(5.0))
+        R = 0.5 * self.D
+        return np.stack([self.cx + R * r * np.cos(ph), self.cy + R * r * np.sin(ph), self.cz + R * z], axis=1)
+
+    def walls(self):
+        w = []
+        if self.inlet == "eq":
+            w.append(dict(kind="eq", mode="plain", loc="left", ux_wall=self.U0))
+        elif self.inlet == "nebb":
+            w.append(dict(kind="nebb", mode="plain", loc="left", ux_wall=self.U0))
+        if self.outlet == "pressure_nee":
+            w.append(dict(kind="nee", mode="pressure", loc="right", rho_wall=1.0))
+        elif self.outlet == "eq":
+            w.append(dict(kind="eq", mode="plain", loc="right", ux_wall=self.U0))
+        return w
+
+    def drag_coefficient(self, fx):
+        """C_d = F_x / (0.5 rho U0^2 A), A = pi D^2 / 4, rho = 1."""
+        return fx / (0.5 * self.U0 ** 2 * math.pi * self.D ** 2 / 4)
+
+
+class Sphere3D:
+    """Device state and fused run of a ``SphereCase`` (``vsx_sphere3d_run``)."""
+
+    def __init__(self, case: SphereCase, storage="f32"):
+        from . import ib3d
+        self.case = case
+        dev = L.device()
+        self.lbm = LBM((case.nx, case.ny, case.nz), case.omega, collision=case.collision, walls=case.walls(),
+                       forcing=case.forcing, storage=storage)
+        S = case.ib_size
+        org = case.ib_origin
+        # stencil of the fixed markers relative to the window origin (flow_past_sphere.py:114-121)
+        rel = torch.as_tensor(case.markers() - np.array(org, dtype=np.float64), dtype=torch.float32, device=dev)
+        self.W, self.I = ib3d.get_ib_stencil(rel, (S, S, S), kernel=case.kernel)
+        cells = S ** 3
+        m, s = self.W.shape
+        self.cell_ptr = torch.empty(cells + 1, dtype=torch.int32, device=dev)
+        self.cell_slot = torch.empty(m * s, dtype=torch.int32, device=dev)
+        tmp = torch.empty(cells + 1, dtype=torch.int32, device=dev)
+        L.call("vsx_ib3d_transpose_stencil", L.ptr(self.I), m, s, cells, L.ptr(self.cell_ptr), L.ptr(self.cell_slot),
+               L.ptr(tmp))
+        self.u_target = torch.zeros((m, 3), dtype=torch.float32, device=dev)
+        self.g_win = torch.zeros(3 * cells, dtype=torch.float32, device=dev)
+        self.work = torch.zeros(int(L.lib().vsx_ib3d_work_floats(m, cells)), dtype=torch.float32, device=dev)
+        ib = L.VsxIB3D()
+        ib.n_markers, ib.n_iter = m, case.n_iter
+        for a in range(3):
+            ib.origin[a] = org[a]
+            ib.win[a] = S
+        ib.W, ib.I = self.W.data_ptr(), self.I.data_ptr()
+        ib.cell_ptr, ib.cell_slot = self.cell_ptr.data_ptr(), self.cell_slot.data_ptr()
+        ib.u_target = self.u_target.data_ptr()
+        ib.ds_arr = None
+        ib.ds = case.ds
+        ib.g_win, ib.work = self.g_win.data_ptr(), self.work.data_ptr()
+        self.ib = ib
+
+    def initial_f(self):
+        """Equilibrium at rho = 1, u = (U0, 0, 0) (flow_past_sphere.py:127-131)."""
+        from . import lbm3d
+        c = self.case
+        u = torch.zeros((3, c.nx, c.ny, c.nz), dtype=torch.float32, device=L.device())
+        u[0] = c.U0
+        return lbm3d.get_equilibrium(torch.ones((c.nx, c.ny, c.nz), dtype=torch.float32, device=L.device()), u)
+
+    def run(self, f, n_steps, out=None, history=True):
+        """Advance n_steps; returns (f_out, h_hist [n, 3] FP64 total marker force on the fluid's markers)."""
+        f = f32(f)
+        out = torch.empty_like(f) if out is None else out
+        hh = None
+        if history and n_steps > 0:
+            hh = torch.empty((n_steps, 3), dtype=torch.float64, device=f.device)
+        self.ib.hist_h = None if hh is None else hh.data_ptr()
+        wa, wb = self.lbm.work()
+        L.call("vsx_sphere3d_run", ctypes.byref(self.lbm.p), ctypes.byref(self.ib), L.ptr(f), L.ptr(out), L.ptr(wa),
+               L.ptr(wb), int(n_steps))
+        return out, hh
