@@ -133,6 +133,29 @@ typedef struct VsxIB3D {
   double *hist_h;             /* optional device [n_steps][3]: total marker force (FP64)        */
 } VsxIB3D;
 
+/* One case of a batched 2D VIV sweep.  All cases of a batch advance together,
+ * one launch per step for the whole batch (IB workgroups of every case plus
+ * the bulk tiles of every case); the step of each case is the one of
+ * vsx_viv2d_run (examples/2d/vortex_induced_vibration.py:96-157). */
+typedef struct VsxVivCase {
+  VsxLbm lbm;          /* per-case grid, omega, walls; collision, forcing and storage equal across the batch */
+  VsxIB2D ib;          /* per-case body; ib.step0 and ib.fused are not used; the histories are indexed
+                          by the batch clock (global step index)                                       */
+  int64_t f_offset;    /* offset, in populations, of the case's f[9][nx][ny] in the batch buffers     */
+  int32_t tile_y;      /* bulk tile = (1024 / tile_y) x tile_y cells; tile_y in {64, 128, 256, 512, 1024} */
+  int32_t hist_cap;    /* steps that ib.hist_d / ib.hist_h can hold                                    */
+} VsxVivCase;
+
+/* A prepared batch (filled by vsx_viv2d_batch_prepare). */
+typedef struct VsxVivBatch {
+  int32_t n_cases, collision, storage, lds_bytes;
+  int64_t n_blocks;    /* workgroups of one step launch                                                 */
+  int64_t total_pops;  /* populations in each batch buffer (f_in, f_out, work_a, work_b)                */
+  void *table;         /* device table, vsx_viv2d_batch_table_bytes bytes, caller-owned                 */
+  int32_t *clock;      /* device int32, caller-owned and zeroed before the first run: global index of
+                          the next step; every run advances it by n_steps                               */
+} VsxVivBatch;
+
 /* ------------------------------------------------------------------ library */
 const char *vsx_version(void);
 const char *vsx_last_error(void);
@@ -245,6 +268,23 @@ int64_t vsx_ib3d_work_floats(int32_t n_markers, int64_t cells);
  * scratch of n_grid + 1 int32. */
 int vsx_ib3d_transpose_stencil(const int32_t *I, int32_t m, int32_t s, int64_t n_grid, int32_t *cell_ptr,
                                int32_t *cell_slot, int32_t *tmp, void *stream);
+
+/* Batched VIV sweep (no counterpart in the reference, which runs one case per
+ * jitted program; each case is the step of vsx_viv2d_run).  prepare validates
+ * the cases, orders the bulk tiles (tiles in the x-band of each case's IB window
+ * last) and uploads the table (a synchronous copy: call it outside graph
+ * capture); run advances every case n_steps; graph captures run into a HIP
+ * graph on a private stream and returns its executable (replay with
+ * vsx_graph_launch: every replay continues from the batch clock). */
+int64_t vsx_viv2d_batch_table_bytes(const VsxVivCase *cases, int32_t n_cases);
+int vsx_viv2d_batch_prepare(const VsxVivCase *cases, int32_t n_cases, void *table, int32_t *clock,
+                            VsxVivBatch *out);
+int vsx_viv2d_batch_run(const VsxVivBatch *b, const float *f_in, float *f_out, void *work_a, void *work_b,
+                        int32_t n_steps, void *stream);
+int vsx_viv2d_batch_graph(const VsxVivBatch *b, const float *f_in, float *f_out, void *work_a, void *work_b,
+                          int32_t n_steps, void **graph_exec);
+int vsx_graph_launch(void *graph_exec, void *stream);
+int vsx_graph_destroy(void *graph_exec);
 
 /* ------------------------------------------------------------------ immersed boundary */
 

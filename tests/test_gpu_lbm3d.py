@@ -2,6 +2,7 @@
 step against the oracle (float64 on the same float32 inputs)."""
 
 import numpy as np
+import torch
 import pytest
 
 from oracle import lbm3d as O

@@ -55,6 +55,15 @@ class VsxIB2D(ctypes.Structure):
                 ("win_origin", c_void_p), ("step0", c_int32), ("fused", c_int32)]
 
 
+class VsxVivCase(ctypes.Structure):
+    _fields_ = [("lbm", VsxLbm), ("ib", VsxIB2D), ("f_offset", c_int64), ("tile_y", c_int32), ("hist_cap", c_int32)]
+
+
+class VsxVivBatch(ctypes.Structure):
+    _fields_ = [("n_cases", c_int32), ("collision", c_int32), ("storage", c_int32), ("lds_bytes", c_int32),
+                ("n_blocks", c_int64), ("total_pops", c_int64), ("table", c_void_p), ("clock", c_void_p)]
+
+
 class VsxIB3D(ctypes.Structure):
     _fields_ = [("n_markers", c_int32), ("n_iter", c_int32), ("origin", c_int32 * 3), ("win", c_int32 * 3),
                 ("W", c_void_p), ("I", c_void_p), ("cell_ptr", c_void_p), ("cell_slot", c_void_p),
@@ -90,6 +99,11 @@ _SIGS = {
     "vsx_viv2d_run": "pppppp i p",
     "vsx_sphere3d_run": "pppppp i p",
     "vsx_ib3d_transpose_stencil": "piilppp p",
+    "vsx_viv2d_batch_prepare": "pippp",
+    "vsx_viv2d_batch_run": "ppppp i p",
+    "vsx_viv2d_batch_graph": "ppppp i p",
+    "vsx_graph_launch": "pp",
+    "vsx_graph_destroy": "p",
 }
 for _pfx in ("d2q9", "d3q19"):
     _SIGS.update({
@@ -117,7 +131,7 @@ _SIGS.update({
 })
 
 # declared in _declare with their own return types
-NON_STATUS = {"vsx_version", "vsx_last_error", "vsx_ib3d_work_floats"}
+NON_STATUS = {"vsx_version", "vsx_last_error", "vsx_ib3d_work_floats", "vsx_viv2d_batch_table_bytes"}
 
 _LIB = None
 
@@ -130,6 +144,8 @@ def _declare(L):
     # entry points that do not return a status code
     L.vsx_ib3d_work_floats.restype = c_int64
     L.vsx_ib3d_work_floats.argtypes = [c_int32, c_int64]
+    L.vsx_viv2d_batch_table_bytes.restype = c_int64
+    L.vsx_viv2d_batch_table_bytes.argtypes = [c_void_p, c_int32]
     for name in ("vsx_version", "vsx_last_error"):
         fn = getattr(L, name)
         fn.restype = ctypes.c_char_p

@@ -39,12 +39,18 @@ struct Timer {
 Timer g_timer;
 }  // namespace
 
+// no event pairs while a stream is being captured into a graph
+static bool capturing(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone;
+}
+
 void timer_begin(hipStream_t s) {
-  if (g_timer.on && g_timer.n < (int)g_timer.a.size()) (void)hipEventRecord(g_timer.a[g_timer.n], s);
+  if (g_timer.on && g_timer.n < (int)g_timer.a.size() && !capturing(s)) (void)hipEventRecord(g_timer.a[g_timer.n], s);
 }
 
 void timer_end(hipStream_t s) {
-  if (g_timer.on && g_timer.n < (int)g_timer.a.size()) {
+  if (g_timer.on && g_timer.n < (int)g_timer.a.size() && !capturing(s)) {
     (void)hipEventRecord(g_timer.b[g_timer.n], s);
     ++g_timer.n;
   }

@@ -196,14 +196,10 @@ class VIVCase:
         return u
 
 
-class VIV2D:
-    """Device state and fused run of a ``VIVCase``."""
+class _Body2D:
+    """Device buffers of one 2D immersed body and the VsxIB2D that points at them."""
 
-    def __init__(self, case: VIVCase, storage="f32", fused=True):
-        self.case = case
-        dev = L.device()
-        self.lbm = LBM((case.nx, case.ny), case.omega, collision=case.collision, walls=case.walls(),
-                       forcing=case.forcing, storage=storage)
+    def __init__(self, case, dev):
         mx, my = case.markers()
         self.marker_x = torch.as_tensor(mx, dtype=torch.float64, device=dev)
         self.marker_y = torch.as_tensor(my, dtype=torch.float64, device=dev)
@@ -213,7 +209,6 @@ class VIV2D:
         self.win_origin = torch.zeros(4, dtype=torch.int32, device=dev)
         # two slots of [a0 a1 v0 v1 d0 d1]: slot (t & 1) holds the state before step t
         self.body = torch.zeros((2, 6), dtype=torch.float64, device=dev)
-        self.t_total = 0
         ib = L.VsxIB2D()
         ib.n_markers = case.n_markers
         ib.kernel = L.KERNEL[case.kernel]
@@ -229,15 +224,31 @@ class VIV2D:
         ib.body = self.body.data_ptr()
         ib.g_win = self.g_win.data_ptr()
         ib.win_origin = self.win_origin.data_ptr()
-        ib.fused = int(bool(fused))
         self.ib = ib
-        self._hist = None
+
+    def set(self, a, v, d, slot=0):
+        self.body.zero_()
+        self.body[slot].copy_(torch.tensor(list(a) + list(v) + list(d), dtype=torch.float64))
+        self.win_origin.zero_()
+
+
+class VIV2D:
+    """Device state and fused run of a ``VIVCase``."""
+
+    def __init__(self, case: VIVCase, storage="f32", fused=True):
+        self.case = case
+        dev = L.device()
+        self.lbm = LBM((case.nx, case.ny), case.omega, collision=case.collision, walls=case.walls(),
+                       forcing=case.forcing, storage=storage)
+        self._b = _Body2D(case, dev)
+        self.body, self.win_origin, self.g_win = self._b.body, self._b.win_origin, self._b.g_win
+        self.t_total = 0
+        self.ib = self._b.ib
+        self.ib.fused = int(bool(fused))
 
     def set_body(self, a=(0.0, 0.0), v=(0.0, 0.0), d=(0.0, 0.0)):
         """Reset the structure state (and the step counter of the body slots)."""
-        self.body.zero_()
-        self.body[0].copy_(torch.tensor(list(a) + list(v) + list(d), dtype=torch.float64))
-        self.win_origin.zero_()
+        self._b.set(a, v, d)
         self.t_total = 0
 
     def state(self):
@@ -397,3 +408,132 @@ class Sphere3D:
         L.call("vsx_sphere3d_run", ctypes.byref(self.lbm.p), ctypes.byref(self.ib), L.ptr(f), L.ptr(out), L.ptr(wa),
                L.ptr(wb), int(n_steps))
         return out, hh
+
+
+# ====================================================================== batched VIV sweep
+
+def default_tile_y(ny):
+    """Bulk tile width along y: the contiguous axis rounded up to a wavefront, at most 256."""
+    return min(256, ((ny + 63) // 64) * 64)
+
+
+class BatchGraph:
+    """A captured HIP graph of a batched run (``vsx_viv2d_batch_graph``); ``launch()``
+    replays it on the current stream, each replay continuing from the batch clock."""
+
+    def __init__(self, handle, refs):
+        self._h = handle
+        self._refs = refs     # keeps the captured buffers alive
+
+    def launch(self):
+        L.call("vsx_graph_launch", self._h)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and L._LIB is not None:
+            L._LIB.vsx_graph_destroy(self._h)
+            self._h = None
+
+
+class VIVBatch:
+    """Many ``VIVCase``s (grids of different sizes, different structures) advanced
+    together, one launch per step for the whole batch (``vsx_viv2d_batch_*``).
+
+    The populations of all cases live in one buffer: case k occupies
+    ``[offset[k], offset[k] + 9 nx ny)`` as its ``f[9, nx, ny]``.
+    ``tile_y``: None (default rule), "auto" (the autotuner in ``vivsim_amd.tune``)
+    or an int applied to every case."""
+
+    def __init__(self, cases, storage="f32", tile_y=None, hist_cap=0):
+        from . import tune
+        cases = list(cases)
+        require(len(cases) > 0, "empty batch")
+        dev = L.device()
+        self.cases, self.storage, self.hist_cap = cases, storage, int(hist_cap)
+        self.lbms = [LBM((c.nx, c.ny), c.omega, collision=c.collision, walls=c.walls(), forcing=c.forcing,
+                         storage=storage) for c in cases]
+        self.bodies = [_Body2D(c, dev) for c in cases]
+        self.offsets, off = [], 0
+        for c in cases:
+            self.offsets.append(off)
+            off += ((9 * c.nx * c.ny + 63) // 64) * 64
+        self.total = off
+        if tile_y == "auto":
+            self.tile_y = [tune.best_tile_y(c, storage) for c in cases]
+        elif tile_y is None:
+            self.tile_y = [default_tile_y(c.ny) for c in cases]
+        else:
+            self.tile_y = [int(tile_y)] * len(cases)
+        self.hist_d = [torch.zeros((max(1, self.hist_cap), 2), dtype=torch.float64, device=dev) for _ in cases]
+        self.hist_h = [torch.zeros((max(1, self.hist_cap), 2), dtype=torch.float64, device=dev) for _ in cases]
+        vc = (L.VsxVivCase * len(cases))()
+        for k, c in enumerate(cases):
+            vc[k].lbm = self.lbms[k].p
+            ib = self.bodies[k].ib
+            ib.hist_d, ib.hist_h = self.hist_d[k].data_ptr(), self.hist_h[k].data_ptr()
+            vc[k].ib = ib
+            vc[k].f_offset = self.offsets[k]
+            vc[k].tile_y = self.tile_y[k]
+            vc[k].hist_cap = self.hist_cap
+        self._vc = vc
+        nbytes = int(L.lib().vsx_viv2d_batch_table_bytes(vc, len(cases)))
+        require(nbytes > 0, "bad batch description")
+        self.table = torch.zeros(nbytes, dtype=torch.uint8, device=dev)
+        self.clock_t = torch.zeros(1, dtype=torch.int32, device=dev)
+        b = L.VsxVivBatch()
+        L.check(L.lib().vsx_viv2d_batch_prepare(vc, len(cases), L.ptr(self.table), L.ptr(self.clock_t),
+                                                ctypes.byref(b)))
+        self.batch = b
+        dt = _STORE_DTYPE[storage]
+        self._work = [torch.empty(self.total, dtype=dt, device=dev) for _ in range(2)]
+
+    @property
+    def cells(self):
+        return sum(c.nx * c.ny for c in self.cases)
+
+    def view(self, f, k):
+        c = self.cases[k]
+        return f[self.offsets[k]:self.offsets[k] + 9 * c.nx * c.ny].view(9, c.nx, c.ny)
+
+    def initial_f(self):
+        """Every case's seeded perturbed inflow at equilibrium, packed into one buffer."""
+        from . import lbm
+        f = torch.zeros(self.total, dtype=torch.float32, device=L.device())
+        for k, c in enumerate(self.cases):
+            u = torch.as_tensor(c.initial_u(), dtype=torch.float32, device=f.device)
+            self.view(f, k).copy_(lbm.get_equilibrium(torch.ones((c.nx, c.ny), device=f.device), u))
+        return f
+
+    def set_bodies(self, v=(0.0, 0.0)):
+        """Every body at rest at its origin with velocity v; the batch clock restarts at 0."""
+        for b in self.bodies:
+            b.set((0.0, 0.0), v, (0.0, 0.0))
+        self.clock_t.zero_()
+
+    def clock(self):
+        return int(self.clock_t.item())
+
+    def state(self, k):
+        b = self.bodies[k].body[self.clock() & 1].cpu().numpy()
+        return b[0:2].copy(), b[2:4].copy(), b[4:6].copy()
+
+    def check(self):
+        for k, b in enumerate(self.bodies):
+            if int(b.win_origin[3].item()) != 0:
+                raise RuntimeError(f"case {k}: window tile timed out waiting for the IB workgroup")
+
+    def run(self, f, n_steps, out=None):
+        f = f32(f)
+        require(f.numel() == self.total, "f must be the packed batch buffer")
+        out = torch.empty_like(f) if out is None else out
+        wa, wb = self._work
+        L.call("vsx_viv2d_batch_run", ctypes.byref(self.batch), L.ptr(f), L.ptr(out), L.ptr(wa), L.ptr(wb),
+               int(n_steps))
+        return out
+
+    def graph(self, f_in, f_out, n_steps):
+        """Capture a run of n_steps from f_in to f_out into a HIP graph."""
+        wa, wb = self._work
+        h = ctypes.c_void_p()
+        L.check(L.lib().vsx_viv2d_batch_graph(ctypes.byref(self.batch), L.ptr(f_in), L.ptr(f_out), L.ptr(wa),
+                                              L.ptr(wb), int(n_steps), ctypes.byref(h)))
+        return BatchGraph(h, (f_in, f_out, self))
