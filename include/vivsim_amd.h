@@ -100,11 +100,13 @@ typedef struct VsxIB2D {
   double dt, gamma, beta;
   double *body;        /* device FP64 [2][6]: slot (step0 + t) & 1 holds [a0 a1 v0 v1 d0 d1]
                           before step t; step t writes the other slot                       */
-  float *g_win;        /* device scratch [2][win][win]                                      */
+  float *g_win;        /* device [2][win][win]: force window of the last step (within a fused
+                          step it first carries the window velocity to the IB workgroup)   */
   double *hist_d, *hist_h;   /* optional device histories [n_steps][2]                      */
-  int32_t *win_origin; /* device int32 [4]: window origin x, y of the last step, ready flag
-                          (= step0 + t + 1 once step t's force window is written), error
-                          flag (1 if a wait for the ready flag ran out)                      */
+  int32_t *win_origin; /* device int32 [8]: [0..1] window origin x, y of the last step;
+                          [2] ready flag (= step0 + t + 1 once step t's force window is
+                          written); [3] error flag (1 if a bounded wait ran out); [4..5]
+                          window-tile counters of the fused step; [6..7] reserved          */
   int32_t step0;       /* global index of the first step of this run (0 after a reset of the
                           body state, which must also zero win_origin)                       */
   int32_t fused;       /* 1: one launch per step, the IB workgroup overlapped with the bulk

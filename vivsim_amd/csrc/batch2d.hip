@@ -1,17 +1,19 @@
 // Batched 2D VIV sweep: every case of a batch advances in one launch per step.
 //
 // Launch layout (grid = n_cases + total tiles, 1024 threads per workgroup):
-//   blocks [0, n_cases)           the IB workgroup of case b (window velocity,
-//                                 multi-direct forcing, FP64 body force, Newmark)
+//   blocks [0, n_cases)           the IB workgroup of case b (multi-direct
+//                                 forcing, FP64 body force, Newmark)
 //   blocks [n_cases, ...)         bulk tiles, through a block map: first the
-//                                 tiles of every case outside the x-band of its
-//                                 IB window, then the tiles inside the bands.
-// A tile that intersects its case's force window waits (bounded) for that
-// case's ready flag; the band tiles come last so that by the time they start
-// the IB workgroups, which start first, are normally done.  Every kernel reads
-// the batch clock (global step index of the run's first step) from device
-// memory, so a captured HIP graph replays correctly: the ready flags, body
-// slots and history rows all follow the clock, which each run advances.
+//                                 tiles of every case inside the x-band of its
+//                                 IB window, then all other tiles.
+// A tile that intersects its case's force window publishes the window velocity
+// of its cells to that case's IB workgroup, waits (bounded) for the case's
+// force window and then collides; the band tiles come first so that the IB
+// workgroups get their input early and run while the other tiles stream.
+// Every kernel reads the batch clock (global step index of the run's first
+// step) from device memory, so a captured HIP graph replays correctly: the
+// ready flags, tile counters, body slots and history rows all follow the
+// clock, which each run advances.
 #include <algorithm>
 #include <cstring>
 #include <type_traits>
@@ -35,29 +37,6 @@ constexpr int kCaseShift = 20;   // block map entry = case << 20 | tile index
 // Populations of one tile cell of case `bc`.  PHASE: COLLIDE reads the
 // caller's post-streaming state in place, FULL / STREAM pull and apply walls.
 template <class SRC, class DST, int COLL, int PHASE>
-__device__ __forceinline__ void batch_cell(const BatchCase &bc, const typename SRC::T *__restrict__ src,
-                                           typename DST::T *__restrict__ dst, const int (&co)[3], const int (&org)[3],
-                                           bool win) {
-  const StepCtx &c = bc.c;
-  const uint32_t cell = cell_of<D2Q9>(c, co[0], co[1], 0);
-  float f[9];
-  if constexpr (PHASE == VSX_PHASE_COLLIDE) {
-    local<D2Q9, SRC>(c, src, cell, f);
-  } else {
-    const int cls = c.p.n_walls > 0 ? wall_class<D2Q9>(c, co) : -1;
-    if (cls >= 0) {
-      single_wall_state<D2Q9, SRC>(c, src, co, cls, f);
-      if constexpr (PHASE != VSX_PHASE_STREAM) finish<D2Q9, COLL>(c, cell, co, org, win, f);
-      store_cell<D2Q9, DST>(c, dst, cell, f);
-      return;
-    }
-    pull<D2Q9, SRC>(c, src, co[0], co[1], 0, f);
-  }
-  if constexpr (PHASE != VSX_PHASE_STREAM) finish<D2Q9, COLL>(c, cell, co, org, win, f);
-  store_cell<D2Q9, DST>(c, dst, cell, f);
-}
-
-template <class SRC, class DST, int COLL, int PHASE>
 __global__ __attribute__((amdgpu_flat_work_group_size(kTileCells, kTileCells), amdgpu_waves_per_eu(8, 8))) void
 k_viv_batch(const BatchCase *__restrict__ tab, const uint32_t *__restrict__ bmap, int n_cases,
             const typename SRC::T *__restrict__ src, typename DST::T *__restrict__ dst, int step,
@@ -67,40 +46,62 @@ k_viv_batch(const BatchCase *__restrict__ tab, const uint32_t *__restrict__ bmap
   const int b = blockIdx.x;
   const int gstep = IB ? *clock + step : 0;
   const int slot = gstep & 1;
-  if (b < n_cases) {
-    if constexpr (IB) {
-      const BatchCase &bc = tab[b];
-      ib_block<SRC, PHASE == VSX_PHASE_FULL, 1, 2, 2>(bc.c, bc.ib, src + bc.f_off, gstep < bc.hist_cap ? gstep : -1,
-                                                      bc.ib.body + 6 * slot, bc.ib.body + 6 * (slot ^ 1), lds);
-      publish_ready(bc.ib, gstep + 1);
-    }
-    return;
-  }
-  const uint32_t e = bmap[b - n_cases];
-  const BatchCase &bc = tab[e >> kCaseShift];
-  const int tile = (int)(e & ((1u << kCaseShift) - 1));
-  const int TX = bc.T.TX, TY = bc.T.TY;
-  const int tx = tile / bc.T.nty, ty = tile % bc.T.nty;
+  const BatchCase &bc = b < n_cases ? tab[b] : tab[bmap[b - n_cases] >> kCaseShift];
+  const StepCtx &c = bc.c;
   int org[3] = {0, 0, 0};
-  bool win = false;
+  int wx0 = 0, wx1 = -1, wy0 = 0, wy1 = -1;
   if constexpr (IB) {
     __shared__ int s_org[2];
     if (threadIdx.x == 0) {
       int o[4];
-      window_origin(bc.c, bc.ib, bc.ib.body + 6 * slot, o);
+      window_origin(c, bc.ib, bc.ib.body + 6 * slot, o);
       s_org[0] = o[2];
       s_org[1] = o[3];
     }
     __syncthreads();
     org[0] = s_org[0];
     org[1] = s_org[1];
-    const int S = bc.ib.win;
-    win = tx * TX < org[0] + S && (tx + 1) * TX > org[0] && ty * TY < org[1] + S && (ty + 1) * TY > org[1];
-    if (win) wait_ready(bc.ib, gstep + 1);
+    window_tiles(bc.T, org[0], org[1], bc.ib.win, wx0, wx1, wy0, wy1);
   }
+  if (b < n_cases) {
+    if constexpr (IB) {
+      ib_block<SRC, PHASE == VSX_PHASE_FULL, 1, 2, 2, true>(c, bc.ib, src + bc.f_off,
+                                                            gstep < bc.hist_cap ? gstep : -1, gstep,
+                                                            (wx1 - wx0 + 1) * (wy1 - wy0 + 1), bc.ib.body + 6 * slot,
+                                                            bc.ib.body + 6 * (slot ^ 1), lds);
+      publish_ready(bc.ib, gstep + 1);
+    }
+    return;
+  }
+  const int tile = (int)(bmap[b - n_cases] & ((1u << kCaseShift) - 1));
+  const int TX = bc.T.TX, TY = bc.T.TY;
+  const int tx = tile / bc.T.nty, ty = tile % bc.T.nty;
+  const bool win = IB && tx >= wx0 && tx <= wx1 && ty >= wy0 && ty <= wy1;
   const int co[3] = {tx * TX + (int)threadIdx.x / TY, ty * TY + (int)threadIdx.x % TY, 0};
-  if (co[0] >= bc.c.dims[0] || co[1] >= bc.c.dims[1]) return;
-  batch_cell<SRC, DST, COLL, PHASE>(bc, src + bc.f_off, dst + bc.f_off, co, org, win);
+  const bool valid = co[0] < c.dims[0] && co[1] < c.dims[1];
+  const uint32_t cell = valid ? cell_of<D2Q9>(c, co[0], co[1], 0) : 0;
+  const typename SRC::T *__restrict__ s_case = src + bc.f_off;
+  typename DST::T *__restrict__ d_case = dst + bc.f_off;
+  float f[9];
+  int cls = -1;
+  if (valid) {
+    if constexpr (PHASE == VSX_PHASE_COLLIDE) {
+      local<D2Q9, SRC>(c, s_case, cell, f);
+    } else {
+      cls = c.p.n_walls > 0 ? wall_class<D2Q9>(c, co) : -1;
+      if (cls >= 0) single_wall_state<D2Q9, SRC>(c, s_case, co, cls, f);
+      else pull<D2Q9, SRC>(c, s_case, co[0], co[1], 0, f);
+    }
+  }
+  if constexpr (IB) {
+    if (win) {
+      publish_window_u(bc.ib, gstep, valid, co, org, f);
+      wait_ready(bc.ib, gstep + 1);
+    }
+  }
+  if (!valid) return;
+  if constexpr (PHASE != VSX_PHASE_STREAM) finish<D2Q9, COLL>(c, cell, co, org, win, f);
+  store_cell<D2Q9, DST>(c, d_case, cell, f);
 }
 
 __global__ void k_clock_advance(int32_t *clock, int n) {
@@ -207,14 +208,14 @@ int batch_prepare(const VsxVivCase *cases, int n, void *table, int32_t *clock, V
         (tx >= bc.band_lo && tx <= bc.band_hi ? band : outer).push_back(entry);
       }
     pops = std::max(pops, vc.f_offset + (int64_t)9 * vc.lbm.nx * vc.lbm.ny);
-    lds = std::max(lds, (int)(sizeof(float) * 2 * vc.ib.win * vc.ib.win));
+    lds = std::max(lds, 4 * ib_lds_words(vc.ib.win, vc.ib.n_markers, true));
   }
   VSX_REQUIRE(lds + 1024 <= max_lds, "IB windows need %d B of LDS, device allows %d", lds, max_lds);
   std::vector<char> host(sizeof(BatchCase) * n + 4 * (outer.size() + band.size()));
   std::memcpy(host.data(), tab.data(), sizeof(BatchCase) * n);
   char *bm = host.data() + sizeof(BatchCase) * n;
-  std::memcpy(bm, outer.data(), 4 * outer.size());
-  std::memcpy(bm + 4 * outer.size(), band.data(), 4 * band.size());
+  std::memcpy(bm, band.data(), 4 * band.size());
+  std::memcpy(bm + 4 * band.size(), outer.data(), 4 * outer.size());
   VSX_HIP(hipMemcpy(table, host.data(), host.size(), hipMemcpyHostToDevice));
   out->n_cases = n;
   out->collision = p0.collision;

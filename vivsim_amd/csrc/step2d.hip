@@ -10,9 +10,10 @@ __global__ __launch_bounds__(kIbThreads) void k_viv_ib(StepCtx c_arg, VsxIB2D ib
                                                         const typename SRC::T *__restrict__ src, int step) {
   const StepCtx &c = kernel_ctx();
   extern __shared__ float lds[];
-  const int slot = (ib.step0 + step) & 1;
-  ib_block<SRC, PULL, MPT, 4, 0>(c, ib, src, step, ib.body + 6 * slot, ib.body + 6 * (slot ^ 1), lds);
-  publish_ready(ib, ib.step0 + step + 1);
+  const int gstep = ib.step0 + step, slot = gstep & 1;
+  ib_block<SRC, PULL, MPT, 4, 0, false>(c, ib, src, step, gstep, 0, ib.body + 6 * slot, ib.body + 6 * (slot ^ 1),
+                                        lds);
+  publish_ready(ib, gstep + 1);
 }
 
 template <class SRC, class DST, int COLL>
@@ -21,14 +22,9 @@ k_viv_fused(StepCtx c_arg, VsxIB2D ib, const typename SRC::T *__restrict__ src, 
             int step, Tiles T) {
   const StepCtx &c = kernel_ctx();
   extern __shared__ float lds[];
-  const int slot = (ib.step0 + step) & 1;
+  const int gstep = ib.step0 + step, slot = gstep & 1;
   const double *body_in = ib.body + 6 * slot;
-  if (blockIdx.x == 0) {
-    ib_block<SRC, true, 1, 2, 1>(c, ib, src, step, body_in, ib.body + 6 * (slot ^ 1), lds);
-    publish_ready(ib, ib.step0 + step + 1);
-    return;
-  }
-  __shared__ int s_org[4];
+  __shared__ int s_org[2];
   if (threadIdx.x == 0) {
     int o[4];
     window_origin(c, ib, body_in, o);
@@ -37,26 +33,46 @@ k_viv_fused(StepCtx c_arg, VsxIB2D ib, const typename SRC::T *__restrict__ src, 
   }
   __syncthreads();
   const int org[3] = {s_org[0], s_org[1], 0};
-  const int S = ib.win;
+  int wx0, wx1, wy0, wy1;
+  window_tiles(T, org[0], org[1], ib.win, wx0, wx1, wy0, wy1);
+  if (blockIdx.x == 0) {
+    ib_block<SRC, true, 1, 2, 1, true>(c, ib, src, step, gstep, (wx1 - wx0 + 1) * (wy1 - wy0 + 1), body_in,
+                                       ib.body + 6 * (slot ^ 1), lds);
+    publish_ready(ib, gstep + 1);
+    return;
+  }
   int tx, ty;
   bool in_win;
-  tile_of_block(blockIdx.x - 1, T, org[0] / T.TX, (org[0] + S - 1) / T.TX, org[1] / T.TY, (org[1] + S - 1) / T.TY,
-                tx, ty, in_win);
-  if (in_win) wait_ready(ib, ib.step0 + step + 1);
+  tile_of_block(blockIdx.x - 1, T, wx0, wx1, wy0, wy1, tx, ty, in_win);
   const int co[3] = {tx * T.TX + (int)threadIdx.x / T.TY, ty * T.TY + (int)threadIdx.x % T.TY, 0};
-  if (co[0] >= c.dims[0] || co[1] >= c.dims[1]) return;
-  const uint32_t cell = cell_of<D2Q9>(c, co[0], co[1], 0);
+  const bool valid = co[0] < c.dims[0] && co[1] < c.dims[1];
+  const uint32_t cell = valid ? cell_of<D2Q9>(c, co[0], co[1], 0) : 0;
   float f[9];
-  const int cls = c.p.n_walls > 0 ? wall_class<D2Q9>(c, co) : -1;
-  if (cls != -1) {
-    if (cls < 0) return;   // corner cells: edge kernel
-    single_wall_state<D2Q9, SRC>(c, src, co, cls, f);
+  if (in_win) {
+    // window tile: post-streaming state -> window velocity for the IB
+    // workgroup -> wait for its force window -> collide
+    if (valid) {
+      const int cls = c.p.n_walls > 0 ? wall_class<D2Q9>(c, co) : -1;
+      if (cls >= 0) single_wall_state<D2Q9, SRC>(c, src, co, cls, f);
+      else pull<D2Q9, SRC>(c, src, co[0], co[1], 0, f);
+    }
+    publish_window_u(ib, gstep, valid, co, org, f);
+    wait_ready(ib, gstep + 1);
+    if (!valid) return;
     finish<D2Q9, COLL>(c, cell, co, org, true, f);
     store_cell<D2Q9, DST>(c, dst, cell, f);
     return;
   }
+  if (!valid) return;
+  const int cls = c.p.n_walls > 0 ? wall_class<D2Q9>(c, co) : -1;
+  if (cls >= 0) {
+    single_wall_state<D2Q9, SRC>(c, src, co, cls, f);
+    finish<D2Q9, COLL>(c, cell, co, org, false, f);
+    store_cell<D2Q9, DST>(c, dst, cell, f);
+    return;
+  }
   pull<D2Q9, SRC>(c, src, co[0], co[1], 0, f);
-  finish<D2Q9, COLL>(c, cell, co, org, true, f);
+  finish<D2Q9, COLL>(c, cell, co, org, false, f);
   store_cell<D2Q9, DST>(c, dst, cell, f);
 }
 
@@ -64,7 +80,7 @@ k_viv_fused(StepCtx c_arg, VsxIB2D ib, const typename SRC::T *__restrict__ src, 
 
 template <class SRC, bool PULL>
 int launch_viv_ib(const StepCtx &c, const VsxIB2D &ib, const void *src, int step, hipStream_t s) {
-  const size_t lds = sizeof(float) * 2 * (size_t)ib.win * ib.win;
+  const size_t lds = 4 * (size_t)ib_lds_words(ib.win, ib.n_markers, ib.n_markers <= kIbThreads);
   auto S = static_cast<const typename SRC::T *>(src);
   if (ib.n_markers <= kIbThreads)
     hipLaunchKernelGGL((k_viv_ib<SRC, PULL, 1>), dim3(1), dim3(kIbThreads), lds, s, c, ib, S, step);
@@ -96,17 +112,11 @@ Tiles make_tiles(const StepCtx &c) {
 template <class ST, int COLL>
 int launch_fused_c(const StepCtx &c, const VsxIB2D &ib, const void *src, void *dst, int step, hipStream_t s) {
   const Tiles T = make_tiles(c);
-  const size_t lds = sizeof(float) * 2 * (size_t)ib.win * ib.win;
+  const size_t lds = 4 * (size_t)ib_lds_words(ib.win, ib.n_markers, true);
   timer_begin(s);
   hipLaunchKernelGGL((k_viv_fused<ST, ST, COLL>), dim3(1 + T.ntx * T.nty), dim3(kTileCells), lds, s, c, ib,
                      static_cast<const typename ST::T *>(src), static_cast<typename ST::T *>(dst), step, T);
   timer_end(s);
-  if (c.p.n_walls > 0 && walls_need_edge<D2Q9>(c.p)) {
-    const int64_t pw = edge_per_wall<D2Q9>(c);
-    const int64_t tot = pw * c.p.n_walls;
-    hipLaunchKernelGGL((k_edge<D2Q9, VSX_PHASE_FULL, ST, ST>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
-                       c, static_cast<const typename ST::T *>(src), static_cast<typename ST::T *>(dst), pw);
-  }
   VSX_CHECK_LAUNCH();
   return VSX_OK;
 }
@@ -144,8 +154,9 @@ int viv_run(const VsxLbm *p, const VsxIB2D *ib, const float *f_in, float *f_out,
     if (f_out != f_in) VSX_HIP(hipMemcpyAsync(f_out, f_in, sizeof(float) * 9 * c.n, hipMemcpyDeviceToDevice, s));
     return VSX_OK;
   }
-  // fused launches need one marker per thread of the IB workgroup
-  const bool fused = ib->fused && ib->n_markers <= kIbThreads;
+  // fused launches need one marker per thread of the IB workgroup, and walls
+  // without corner cells (every window cell is then finished by its tile)
+  const bool fused = ib->fused && ib->n_markers <= kIbThreads && !(p->n_walls > 0 && walls_need_edge<D2Q9>(*p));
   const void *cur = f_in;
   void *bufs[2] = {wa, wb};
   for (int t = 0; t < n; ++t) {

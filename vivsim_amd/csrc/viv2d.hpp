@@ -44,28 +44,140 @@ __device__ __noinline__ float2 window_wall_u(const StepCtx &c, const typename SR
 }
 
 // ------------------------------------------------------------------ the IB workgroup
+// LDS of the IB workgroup in 4-byte words (S = window side, M = markers):
+//   ug    [2][S*S]   window velocity, then the spread force
+//   with BINS (one marker per thread), from a 16-byte aligned offset:
+//   rec   [M][12]    base, -, wx0..3, wy0..3, fx, fy of the markers in bin order
+//                    (base: flat window index of stencil point (0, 0); fx, fy:
+//                    the force being spread)
+//   bptr  [S*S + 1]  markers binned by base (CSR: after placement bptr[b] = end of bin b)
+//   blist [M]        markers by bin (placement order)
+//   pos   [M]        bin-ordered position of each marker
+__host__ __device__ inline int ib_rec_offset(int S) { return (2 * S * S + 3) & ~3; }
+__host__ __device__ inline int ib_lds_words(int S, int M, bool bins) {
+  return bins ? ib_rec_offset(S) + 14 * M + S * S + 1 : 2 * S * S;
+}
+
+// Exclusive prefix sum of a[0..n) in place by the whole workgroup.
+__device__ __forceinline__ void block_exclusive_scan(int *a, int n, int *s_wsum) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int per = (n + nt - 1) / nt;
+  const int b0 = min(tid * per, n), b1 = min(b0 + per, n);
+  int sum = 0;
+  for (int b = b0; b < b1; ++b) sum += a[b];
+  int v = sum;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(v, o, 64);
+    if ((tid & 63) >= o) v += t;
+  }
+  if ((tid & 63) == 63) s_wsum[tid >> 6] = v;
+  __syncthreads();
+  if (tid == 0) {
+    int run = 0;
+    for (int w = 0; w < (nt + 63) / 64; ++w) {
+      const int x = s_wsum[w];
+      s_wsum[w] = run;
+      run += x;
+    }
+  }
+  __syncthreads();
+  int run = s_wsum[tid >> 6] + v - sum;
+  for (int b = b0; b < b1; ++b) {
+    const int x = a[b];
+    a[b] = run;
+    run += x;
+  }
+  __syncthreads();
+}
+
+// Spread of the binned markers onto the window as a gather: window cell k
+// receives from the markers whose base lies in the 4 runs of 4 bins
+// k - a S - 3 .. k - a S (a = 0..3), in ascending bin and marker order, with
+// weight wx[a] * wy[k - a S - base].  Every cell is written exactly once.
+__device__ __forceinline__ void bin_spread(float *ug, const float *rec, const int *bptr, int S) {
+  const int SS = S * S;
+  for (int k = threadIdx.x; k < SS; k += blockDim.x) {
+    int e0[4], e1[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int hi = k - a * S, lo = hi - 3;
+      e0[a] = hi >= 0 && lo > 0 ? bptr[lo - 1] : 0;
+      e1[a] = hi >= 0 ? bptr[hi] : 0;
+    }
+    float gx = 0.f, gy = 0.f;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int hi = k - a * S;
+#pragma unroll 2
+      for (int e = e0[a]; e < e1[a]; ++e) {
+        const float4 q0 = *reinterpret_cast<const float4 *>(rec + 12 * e);
+        const float4 q1 = *reinterpret_cast<const float4 *>(rec + 12 * e + 4);
+        const float4 q2 = *reinterpret_cast<const float4 *>(rec + 12 * e + 8);
+        const int b = hi - __float_as_int(q0.x);
+        const float wxa = a == 0 ? q0.z : a == 1 ? q0.w : a == 2 ? q1.x : q1.y;
+        float wyb = q1.z;   // flat selects, no branches
+        wyb = b == 1 ? q1.w : wyb;
+        wyb = b == 2 ? q2.x : wyb;
+        wyb = b == 3 ? q2.y : wyb;
+        const float w = wxa * wyb;
+        gx += q2.z * w;
+        gy += q2.w * w;
+      }
+    }
+    ug[k] = gx;
+    ug[SS + k] = gy;
+  }
+}
+
+// Bounded wait of thread 0 until *flag >= target (agent-scope acquire); on
+// time-out the error flag ib.win_origin[3] is set and the wait ends.
+__device__ __forceinline__ void spin_until(const VsxIB2D &ib, int32_t *flag, int target) {
+  int spins = 0;
+  while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    __builtin_amdgcn_s_sleep(2);
+    if (++spins > kSpinLimit) {
+      __hip_atomic_store(&ib.win_origin[3], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      break;
+    }
+  }
+}
+
 // One workgroup per body (examples/2d/vortex_induced_vibration.py:99-137):
-//   window u     macroscopic velocity of the post-streaming state (pull + walls), in LDS
+//   window u     macroscopic velocity of the post-streaming state (pull + walls);
+//                U_GLOBAL: published in ib.g_win by the n_wtiles tiles that
+//                cover the window (counter ib.win_origin[4 + (gstep & 1)])
 //   stencil      ib/stencil.py:7-51 at marker coordinates (x_m + d) - origin (4x4)
-//   MDF          ib/mdf.py:35-64: interpolate, n_iter x (spread dF into LDS with LDS
-//                atomics, interpolate 0.5 * spread), final spread of the total force
+//   MDF          ib/mdf.py:35-64: interpolate, n_iter x (spread dF, interpolate
+//                0.5 * spread), final spread of the total force.  BINS: spread is
+//                a gather over the markers binned by stencil base cell (see
+//                bin_spread): no atomics, deterministic.
+//                Markers whose stencil leaves the window, and the path with
+//                several markers per thread, use LDS atomics with the scatter
+//                rule of ib/stencil.py:106-110
 //   body force   h = sum_m (-F_m) + a * area in FP64 (wave shuffles + LDS), dyn.py:126-136
 //   structure    Newmark-beta in FP64 on lane 0, dyn.py:5-51
-// `lds` holds 2 * win^2 floats: the window velocity, reused for the spread
-// force once the initial interpolation is done.  Writes the force window to
-// ib.g_win and the new body state to body_out.
-template <class SRC, bool PULL, int MPT, int UW, int TAG>
+// Writes the force window to ib.g_win and the new body state to body_out.
+template <class SRC, bool PULL, int MPT, int UW, int TAG, bool U_GLOBAL>
 __device__ __forceinline__ void ib_block(const StepCtx &c, const VsxIB2D &ib, const typename SRC::T *__restrict__ src,
-                                         int hidx, const double *body_in, double *body_out, float *lds) {
-  const int S = ib.win, SS = S * S;
-  float *ul = lds;   // [2][SS] window velocity, then the spread force
-  float *gl = lds;
+                                         int hidx, int gstep, int n_wtiles, const double *body_in, double *body_out,
+                                         float *lds) {
+  const int S = ib.win, SS = S * S, M = ib.n_markers;
+  constexpr bool BINS = MPT == 1;
+  float *ug = lds;
+  float *rec = lds + ib_rec_offset(S);
+  int *bptr = reinterpret_cast<int *>(rec + 12 * M);
+  int *blist = bptr + SS + 1;
+  int *pos = blist + M;
   __shared__ double s_body[6];
   __shared__ int s_org[4];
+  __shared__ int s_irregular;
+  __shared__ int s_wsum[kIbThreads / 64];
   __shared__ double s_red[2][kIbThreads / 64];
   const int tid = threadIdx.x, nt = blockDim.x;
 
   if (tid < 6) s_body[tid] = body_in[tid];
+  if (tid == 0) s_irregular = 0;
   __syncthreads();
   if (tid == 0) {
     int o[4];
@@ -78,47 +190,12 @@ __device__ __forceinline__ void ib_block(const StepCtx &c, const VsxIB2D &ib, co
   __syncthreads();
   const int x0 = s_org[0], y0 = s_org[1], xc = s_org[2], yc = s_org[3];
 
-  // window velocity: UW cells per thread per pass, all loads issued before use
-  for (int base = 0; base < SS; base += UW * nt) {
-    float f[UW][9];
-    bool on_w[UW];
-#pragma unroll
-    for (int j = 0; j < UW; ++j) {
-      const int k = base + j * nt + tid;
-      on_w[j] = false;
-      if (k < SS) {
-        const int co[3] = {xc + k / S, yc + k % S, 0};
-        if constexpr (!PULL) {
-          local<D2Q9, SRC>(c, src, cell_of<D2Q9>(c, co[0], co[1], 0), f[j]);
-        } else {
-          on_w[j] = c.p.n_walls > 0 && wall_class<D2Q9>(c, co) != -1;
-          if (!on_w[j]) pull<D2Q9, SRC>(c, src, co[0], co[1], 0, f[j]);
-        }
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < UW; ++j) {
-      const int k = base + j * nt + tid;
-      if (k < SS) {
-        if (on_w[j]) {
-          const float2 uw = window_wall_u<SRC, TAG>(c, src, xc + k / S, yc + k % S);
-          ul[k] = uw.x;
-          ul[SS + k] = uw.y;
-        } else {
-          float rho, u[3];
-          macroscopic<D2Q9>(f[j], rho, u);
-          ul[k] = u[0];
-          ul[SS + k] = u[1];
-        }
-      }
-    }
-  }
-
-  // Marker q of this thread: consecutive lanes of a wavefront hold markers nt/64
-  // apart along the body, so the 64 lanes of one LDS atomic (spread) or read
-  // (interpolate) touch different cells instead of serialising on shared ones.
+  // Markers first (they need no field data).  Marker q of this thread:
+  // consecutive lanes of a wavefront hold markers nt/64 apart along the body,
+  // so the lanes of one LDS access touch different cells.
   const float Ux = (float)s_body[2], Uy = (float)s_body[3];
-  int mk[MPT], base_idx[MPT];
+  int mk[MPT], base_idx[MPT], bx[MPT], by[MPT];
+  bool regular[MPT];
   float wx[MPT][4], wy[MPT][4];
   float ubx[MPT], uby[MPT], Fx[MPT], Fy[MPT], ds2[MPT];
   const int nwave = nt >> 6;
@@ -126,10 +203,11 @@ __device__ __forceinline__ void ib_block(const StepCtx &c, const VsxIB2D &ib, co
   for (int q = 0; q < MPT; ++q) {
     mk[q] = q * nt + (tid & 63) * nwave + (tid >> 6);
     ubx[q] = uby[q] = Fx[q] = Fy[q] = ds2[q] = 0.f;
-    base_idx[q] = 0;
+    base_idx[q] = bx[q] = by[q] = 0;
+    regular[q] = false;
 #pragma unroll
     for (int i = 0; i < 4; ++i) wx[q][i] = wy[q][i] = 0.f;
-    if (mk[q] < ib.n_markers) {
+    if (mk[q] < M) {
       const int m = mk[q];
       const float rx = (float)(ib.marker_x[m] + s_body[4] - (double)x0);
       const float ry = (float)(ib.marker_y[m] + s_body[5] - (double)y0);
@@ -139,8 +217,88 @@ __device__ __forceinline__ void ib_block(const StepCtx &c, const VsxIB2D &ib, co
         wx[q][i] = ax.w[i];
         wy[q][i] = ay.w[i];
       }
-      base_idx[q] = (ax.ref - 1) * S + (ay.ref - 1);   // flat window index of stencil point (0, 0)
+      bx[q] = ax.ref - 1;   // window row / column of stencil point (0, 0)
+      by[q] = ay.ref - 1;
+      base_idx[q] = bx[q] * S + by[q];   // its flat window index
       ds2[q] = (ib.ds_arr ? ib.ds_arr[m] : ib.ds) * 2.0f;
+      regular[q] = BINS && bx[q] >= 0 && bx[q] <= S - 4 && by[q] >= 0 && by[q] <= S - 4;
+      if (!regular[q]) s_irregular = 1;
+    }
+  }
+  if constexpr (BINS) {
+    // regular markers binned by base cell: count, scan, place; a marker's
+    // position = start of its bin + its rank by marker index within the bin
+    for (int k = tid; k <= SS; k += nt) bptr[k] = 0;
+    __syncthreads();
+    if (mk[0] < M && regular[0]) atomicAdd(&bptr[base_idx[0]], 1);
+    __syncthreads();
+    block_exclusive_scan(bptr, SS, s_wsum);
+    if (mk[0] < M && regular[0]) blist[atomicAdd(&bptr[base_idx[0]], 1)] = mk[0];
+    __syncthreads();
+    if (mk[0] < M && regular[0]) {
+      const int s0 = base_idx[0] > 0 ? bptr[base_idx[0] - 1] : 0, e0 = bptr[base_idx[0]];
+      int rank = 0;
+      for (int i = s0; i < e0; ++i) rank += blist[i] < mk[0];
+      pos[mk[0]] = s0 + rank;
+    }
+    __syncthreads();
+    if (mk[0] < M && regular[0]) {
+      const int e = pos[mk[0]];
+      float *r = rec + 12 * e;
+      r[0] = __int_as_float(base_idx[0]);
+      r[1] = 0.f;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        r[2 + i] = wx[0][i];
+        r[6 + i] = wy[0][i];
+      }
+    }
+  }
+
+  // window velocity
+  if constexpr (U_GLOBAL) {
+    if (tid == 0) {
+      int32_t *cnt = &ib.win_origin[4 + (gstep & 1)];
+      spin_until(ib, cnt, n_wtiles);
+      __hip_atomic_store(cnt, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);   // reused two steps later
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    for (int k = tid; k < 2 * SS; k += nt) ug[k] = ib.g_win[k];
+  } else {
+    for (int base = 0; base < SS; base += UW * nt) {
+      float f[UW][9];
+      bool on_w[UW];
+#pragma unroll
+      for (int j = 0; j < UW; ++j) {
+        const int k = base + j * nt + tid;
+        on_w[j] = false;
+        if (k < SS) {
+          const int co[3] = {xc + k / S, yc + k % S, 0};
+          if constexpr (!PULL) {
+            local<D2Q9, SRC>(c, src, cell_of<D2Q9>(c, co[0], co[1], 0), f[j]);
+          } else {
+            on_w[j] = c.p.n_walls > 0 && wall_class<D2Q9>(c, co) != -1;
+            if (!on_w[j]) pull<D2Q9, SRC>(c, src, co[0], co[1], 0, f[j]);
+          }
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < UW; ++j) {
+        const int k = base + j * nt + tid;
+        if (k < SS) {
+          if (on_w[j]) {
+            const float2 uw = window_wall_u<SRC, TAG>(c, src, xc + k / S, yc + k % S);
+            ug[k] = uw.x;
+            ug[SS + k] = uw.y;
+          } else {
+            float rho, u[3];
+            macroscopic<D2Q9>(f[j], rho, u);
+            ug[k] = u[0];
+            ug[SS + k] = u[1];
+          }
+        }
+      }
     }
   }
   __syncthreads();
@@ -148,7 +306,7 @@ __device__ __forceinline__ void ib_block(const StepCtx &c, const VsxIB2D &ib, co
   // initial interpolation of the window velocity
 #pragma unroll
   for (int q = 0; q < MPT; ++q) {
-    if (mk[q] < ib.n_markers) {
+    if (mk[q] < M) {
       float sx = 0.f, sy = 0.f;
 #pragma unroll
       for (int i = 0; i < 4; ++i)
@@ -156,8 +314,8 @@ __device__ __forceinline__ void ib_block(const StepCtx &c, const VsxIB2D &ib, co
         for (int j = 0; j < 4; ++j) {
           const int gi = (int)gather_index(base_idx[q] + i * S + j, SS);
           const float w = wx[q][i] * wy[q][j];
-          sx += w * ul[gi];
-          sy += w * ul[SS + gi];
+          sx += w * ug[gi];
+          sy += w * ug[SS + gi];
         }
       ubx[q] = sx;
       uby[q] = sy;
@@ -182,29 +340,42 @@ __device__ __forceinline__ void ib_block(const StepCtx &c, const VsxIB2D &ib, co
         sfy[q] = Fy[q];
       }
     }
-    for (int k = tid; k < 2 * SS; k += nt) gl[k] = 0.f;
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < MPT; ++q) {
-      if (mk[q] < ib.n_markers) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            const int si = (int)scatter_index(base_idx[q] + i * S + j, SS);
-            if (si >= 0) {
-              const float w = wx[q][i] * wy[q][j];
-              atomicAdd(&gl[si], sfx[q] * w);
-              atomicAdd(&gl[SS + si], sfy[q] * w);
-            }
-          }
+    if constexpr (BINS) {
+      if (mk[0] < M && regular[0]) {
+        float *r = rec + 12 * pos[mk[0]];
+        r[10] = sfx[0];
+        r[11] = sfy[0];
       }
+      __syncthreads();
+      bin_spread(ug, rec, bptr, S);
+      __syncthreads();
+    } else {
+      for (int k = tid; k < 2 * SS; k += nt) ug[k] = 0.f;
+      __syncthreads();
     }
-    __syncthreads();
+    if (s_irregular) {
+#pragma unroll
+      for (int q = 0; q < MPT; ++q) {
+        if (mk[q] < M && !regular[q]) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const int si = (int)scatter_index(base_idx[q] + i * S + j, SS);
+              if (si >= 0) {
+                const float w = wx[q][i] * wy[q][j];
+                atomicAdd(&ug[si], sfx[q] * w);
+                atomicAdd(&ug[SS + si], sfy[q] * w);
+              }
+            }
+        }
+      }
+      __syncthreads();
+    }
     if (last) break;
 #pragma unroll
     for (int q = 0; q < MPT; ++q) {
-      if (mk[q] < ib.n_markers) {
+      if (mk[q] < M) {
         float sx = 0.f, sy = 0.f;
 #pragma unroll
         for (int i = 0; i < 4; ++i)
@@ -212,8 +383,8 @@ __device__ __forceinline__ void ib_block(const StepCtx &c, const VsxIB2D &ib, co
           for (int j = 0; j < 4; ++j) {
             const int gi = (int)gather_index(base_idx[q] + i * S + j, SS);
             const float w = wx[q][i] * wy[q][j];
-            sx += w * (gl[gi] * 0.5f);
-            sy += w * (gl[SS + gi] * 0.5f);
+            sx += w * (ug[gi] * 0.5f);
+            sy += w * (ug[SS + gi] * 0.5f);
           }
         ubx[q] += sx;
         uby[q] += sy;
@@ -223,7 +394,7 @@ __device__ __forceinline__ void ib_block(const StepCtx &c, const VsxIB2D &ib, co
   }
 
   // force window -> global
-  for (int k = tid; k < 2 * SS; k += nt) ib.g_win[k] = gl[k];
+  for (int k = tid; k < 2 * SS; k += nt) ib.g_win[k] = ug[k];
 
   // body force h = sum(-F) (FP64)
   double hx = 0.0, hy = 0.0;
@@ -281,60 +452,80 @@ __device__ __forceinline__ void publish_ready(const VsxIB2D &ib, int value) {
   }
 }
 
-// Bounded wait of a window tile (thread 0) until the ready flag reaches
-// `target`; on time-out sets the error flag and proceeds.  Then every thread
-// of the tile acquires at agent scope.
+// Wait of a window tile until the ready flag reaches `target` (bounded, see
+// spin_until); then every thread of the tile acquires at agent scope.
 __device__ __forceinline__ void wait_ready(const VsxIB2D &ib, int target) {
-  if (threadIdx.x == 0) {
-    int spins = 0;
-    while (__hip_atomic_load(&ib.win_origin[2], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      __builtin_amdgcn_s_sleep(2);
-      if (++spins > kSpinLimit) {
-        __hip_atomic_store(&ib.win_origin[3], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        break;
-      }
-    }
-  }
+  if (threadIdx.x == 0) spin_until(ib, &ib.win_origin[2], target);
   __syncthreads();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+// Window tile: publish the post-streaming velocity of its cells that lie in
+// the force window (into ib.g_win, read by the IB workgroup), then count the
+// tile in ib.win_origin[4 + (gstep & 1)] with an agent-scope release.
+__device__ __forceinline__ void publish_window_u(const VsxIB2D &ib, int gstep, bool valid, const int (&co)[3],
+                                                 const int (&org)[3], const float (&f)[9]) {
+  const int S = ib.win;
+  const int r0 = co[0] - org[0], r1 = co[1] - org[1];
+  if (valid && r0 >= 0 && r0 < S && r1 >= 0 && r1 < S) {
+    float rho, u[3];
+    macroscopic<D2Q9>(f, rho, u);
+    ib.g_win[r0 * S + r1] = u[0];
+    ib.g_win[S * S + r0 * S + r1] = u[1];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    __hip_atomic_fetch_add(&ib.win_origin[4 + (gstep & 1)], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
 // ------------------------------------------------------------------ fused VIV step
 // One launch per step.  Block 0 runs the IB workgroup; blocks 1.. are bulk
 // tiles of kTileCells cells (TX rows x TY cells).  Every tile computes the
-// step's window origin from the body state of the previous step; the tiles
-// that intersect the force window are scheduled last and wait (bounded) for
-// the IB workgroup's ready flag, all other tiles start immediately, so the IB
-// work overlaps the streaming of the rest of the grid.
+// step's window origin from the body state before the step.  The tiles that
+// intersect the force window come first: they pull their cells, hand the
+// window velocity to the IB workgroup, wait (bounded) for its force window and
+// then collide; all other tiles stream and collide at once, so the IB work
+// overlaps the streaming of the rest of the grid.
 constexpr int kTileCells = 1024;
 
 struct Tiles {
   int TX, TY, ntx, nty;
 };
 
-__device__ __forceinline__ void tile_of_block(int p, const Tiles &T, int wx0, int wx1, int wy0, int wy1, int &tx,
+// Window tile rectangle [wx0, wx1] x [wy0, wy1] (tile coordinates) of a window at org.
+__host__ __device__ inline void window_tiles(const Tiles &T, int org0, int org1, int S, int &wx0, int &wx1, int &wy0,
+                                             int &wy1) {
+  wx0 = org0 / T.TX;
+  wx1 = (org0 + S - 1) / T.TX;
+  wy0 = org1 / T.TY;
+  wy1 = (org1 + S - 1) / T.TY;
+}
+
+// Block p (0-based among the tile blocks) -> tile (tx, ty): the window
+// rectangle's tiles first, then the others in row-major order.
+__host__ __device__ inline void tile_of_block(int p, const Tiles &T, int wx0, int wx1, int wy0, int wy1, int &tx,
                                               int &ty, bool &in_win) {
   const int nwx = wx1 - wx0 + 1, nwy = wy1 - wy0 + 1;
-  const int nn = T.ntx * T.nty - nwx * nwy;
-  in_win = p >= nn;
+  in_win = p < nwx * nwy;
   if (in_win) {
-    const int q = p - nn;
-    tx = wx0 + q / nwy;
-    ty = wy0 + q % nwy;
+    tx = wx0 + p / nwy;
+    ty = wy0 + p % nwy;
     return;
   }
+  const int q = p - nwx * nwy;
   const int A = wx0 * T.nty, rw = T.nty - nwy, B = nwx * rw;
-  if (p < A) {
-    tx = p / T.nty;
-    ty = p % T.nty;
-  } else if (p < A + B) {
-    const int q = p - A, r = q % rw;
-    tx = wx0 + q / rw;
+  if (q < A) {
+    tx = q / T.nty;
+    ty = q % T.nty;
+  } else if (q < A + B) {
+    const int r = (q - A) % rw;
+    tx = wx0 + (q - A) / rw;
     ty = r < wy0 ? r : r + nwy;
   } else {
-    const int q = p - A - B;
-    tx = wx1 + 1 + q / T.nty;
-    ty = q % T.nty;
+    tx = wx1 + 1 + (q - A - B) / T.nty;
+    ty = (q - A - B) % T.nty;
   }
 }
 
@@ -353,7 +544,7 @@ inline int validate_ib2d(const VsxLbm *p, const VsxIB2D *ib) {
   int dev = 0, max_lds = 0;
   VSX_HIP(hipGetDevice(&dev));
   VSX_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-  const size_t need = sizeof(float) * 2 * (size_t)ib->win * ib->win + 1024;
+  const size_t need = 4 * (size_t)ib_lds_words(ib->win, ib->n_markers, ib->n_markers <= kIbThreads) + 2048;
   VSX_REQUIRE(need <= (size_t)max_lds, "IB window %d needs %zu B of LDS, device allows %d", ib->win, need, max_lds);
   return VSX_OK;
 }

@@ -205,8 +205,8 @@ class _Body2D:
         self.marker_y = torch.as_tensor(my, dtype=torch.float64, device=dev)
         S = case.ib_size
         self.g_win = torch.zeros(2 * S * S, dtype=torch.float32, device=dev)
-        # origin x, y of the last step's window, ready flag, spin-timeout flag
-        self.win_origin = torch.zeros(4, dtype=torch.int32, device=dev)
+        # origin x, y of the last step's window, ready flag, spin-timeout flag, window-tile counters
+        self.win_origin = torch.zeros(8, dtype=torch.int32, device=dev)
         # two slots of [a0 a1 v0 v1 d0 d1]: slot (t & 1) holds the state before step t
         self.body = torch.zeros((2, 6), dtype=torch.float64, device=dev)
         ib = L.VsxIB2D()
