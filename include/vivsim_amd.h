@@ -158,6 +158,31 @@ typedef struct VsxVivBatch {
                           the next step; every run advances it by n_steps                               */
 } VsxVivBatch;
 
+/* Reverse-mode (adjoint) run of the 2D VIV step with sqrt(T) checkpointing.
+ * The reference differentiates its jitted step with jax.grad; here the
+ * backward pass is hand written.  Objective J = (1/T) sum_{t=1..T} d_y(t)^2;
+ * parameters: marker radial offsets delta_m (the markers of the VsxIB2D are
+ * X_m = X0_m + delta_m (dir_x_m, dir_y_m)), the spring stiffness and the inlet
+ * velocity of the equilibrium inlet.  Supported step: D2Q9 MRT + Guo forcing,
+ * walls [equilibrium inlet at left, pressure-NEE outlet at right], one marker
+ * per thread of the IB workgroup (M <= 1024).  All buffers are caller-owned. */
+typedef struct VsxAdj2D {
+  int32_t n_steps, seg;     /* T and the checkpoint interval (steps per recomputed segment)       */
+  const double *dir_x, *dir_y;   /* device [M]: d X_m / d delta_m                                  */
+  float *ckpt;              /* device [ceil(T/seg)][9][n]: post-streaming checkpoints              */
+  double *ckpt_body;        /* device [ceil(T/seg)][6]                                              */
+  float *tape_f;            /* device [seg + 1][9][n]: post-streaming states of a segment           */
+  float *tape_g;            /* device [seg][2][S][S]: force windows of a segment                    */
+  double *tape_body;        /* device [seg + 1][6]: body states of a segment                        */
+  float *adj_a, *adj_b;     /* device [9][n]: adjoint fields                                       */
+  float *grid;              /* device [2][S][S]: adjoint of the force window                        */
+  double *adj_body;         /* device [8]: adjoint of (a, v, d) and the stiffness accumulator        */
+  double *hist_d;           /* device [T][2]: displacement history of the forward run              */
+  double *u_in_part;        /* device [T]: per-step inlet-velocity gradient terms                    */
+  double *grad_delta;       /* device [M] output: dJ / d delta_m                                    */
+  double *out;              /* device [3] output: J, dJ / d stiffness, dJ / d u_in                    */
+} VsxAdj2D;
+
 /* ------------------------------------------------------------------ library */
 const char *vsx_version(void);
 const char *vsx_last_error(void);
@@ -287,6 +312,13 @@ int vsx_viv2d_batch_graph(const VsxVivBatch *b, const float *f_in, float *f_out,
                           int32_t n_steps, void **graph_exec);
 int vsx_graph_launch(void *graph_exec, void *stream);
 int vsx_graph_destroy(void *graph_exec);
+
+/* Gradient of the VIV objective (see VsxAdj2D) for the run f0 -> step^T(f0) that
+ * vsx_viv2d_run would perform with p and ib (ib->step0 and ib->fused are set by the
+ * driver; ib->body slot 0 holds the initial body state).  work_a / work_b as for
+ * vsx_viv2d_run; f_tmp: device [9][n] float32 scratch. */
+int vsx_viv2d_adjoint(const VsxLbm *p, const VsxIB2D *ib, const VsxAdj2D *adj, const float *f0, void *work_a,
+                      void *work_b, float *f_tmp, void *stream);
 
 /* ------------------------------------------------------------------ immersed boundary */
 

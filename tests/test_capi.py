@@ -28,7 +28,7 @@ def test_library_exports_header_symbols():
 def test_struct_layout_matches_c(tmp_path):
     src = tmp_path / "layout.c"
     fields = {"VsxWall": L.VsxWall, "VsxLbm": L.VsxLbm, "VsxIB2D": L.VsxIB2D, "VsxIB3D": L.VsxIB3D,
-              "VsxVivCase": L.VsxVivCase, "VsxVivBatch": L.VsxVivBatch}
+              "VsxVivCase": L.VsxVivCase, "VsxVivBatch": L.VsxVivBatch, "VsxAdj2D": L.VsxAdj2D}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "vivsim_amd.h"', "int main(void) {"]
     for sname, cls in fields.items():
         lines.append(f'printf("{sname} %zu\\n", sizeof({sname}));')

@@ -64,6 +64,14 @@ class VsxVivBatch(ctypes.Structure):
                 ("n_blocks", c_int64), ("total_pops", c_int64), ("table", c_void_p), ("clock", c_void_p)]
 
 
+class VsxAdj2D(ctypes.Structure):
+    _fields_ = [("n_steps", c_int32), ("seg", c_int32), ("dir_x", c_void_p), ("dir_y", c_void_p),
+                ("ckpt", c_void_p), ("ckpt_body", c_void_p), ("tape_f", c_void_p), ("tape_g", c_void_p),
+                ("tape_body", c_void_p), ("adj_a", c_void_p), ("adj_b", c_void_p), ("grid", c_void_p),
+                ("adj_body", c_void_p), ("hist_d", c_void_p), ("u_in_part", c_void_p), ("grad_delta", c_void_p),
+                ("out", c_void_p)]
+
+
 class VsxIB3D(ctypes.Structure):
     _fields_ = [("n_markers", c_int32), ("n_iter", c_int32), ("origin", c_int32 * 3), ("win", c_int32 * 3),
                 ("W", c_void_p), ("I", c_void_p), ("cell_ptr", c_void_p), ("cell_slot", c_void_p),
@@ -104,6 +112,7 @@ _SIGS = {
     "vsx_viv2d_batch_graph": "ppppp i p",
     "vsx_graph_launch": "pp",
     "vsx_graph_destroy": "p",
+    "vsx_viv2d_adjoint": "ppppppp p",
 }
 for _pfx in ("d2q9", "d3q19"):
     _SIGS.update({

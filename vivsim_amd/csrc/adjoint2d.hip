@@ -1,0 +1,591 @@
+// Hand-written reverse mode of the 2D VIV step (MRT + Guo, equilibrium inlet,
+// pressure-NEE outlet, IB multi-direct forcing, FP64 Newmark) with sqrt(T)
+// checkpointing.  Objective J = (1/T) sum_{t=1..T} d_y(t)^2.
+//
+// Forward state of the reference: f_t (post-streaming, post-walls).  Step t:
+//   P_t     = f_t + A (feq(rho, u*) - f_t) + B F(g_t, u*)       (A = M^-1 S M,
+//             B = M^-1 (I - S/2) M, u* = u + g / (2 rho); forcing/guo.py:62-107)
+//   s       = streaming(P_t)                                   (lbm/basic.py:44-85)
+//   f_{t+1} = walls(s): left := feq(1, (u_in, 0)); right := feq(1, (u_n, u_t))
+//             + s_nb - feq(macro(s_nb))                          (eq.py:25-56, nee.py:19-65,
+//                                                                 _helpers.py:98-132)
+//   g_t, H_t = MDF(u(f_t), W(x_m + d_t)), body_{t+1} = Newmark(body_t, H_t)
+// Reverse step t, given fbar_{t+1} and the adjoint of body_{t+1}:
+//   k_adj_walls  wall adjoint in place on fbar_{t+1} (-> sbar), inlet-velocity term
+//   k_adj_bulk   pbar_i(x) = sbar_i(x + c_i); per-cell transpose of the collision +
+//                forcing Jacobian -> fbar_t, and gbar_t on the force window
+//   k_adj_ib     Newmark adjoint, replay + reverse of the MDF loop (transposes of
+//                interpolate / spread, kernel derivative phi' for the marker
+//                positions), window-velocity adjoint added to fbar_t
+#include <algorithm>
+
+#include "step_impl.hpp"
+#include "viv2d.hpp"
+
+namespace vsx {
+
+int viv_run(const VsxLbm *p, const VsxIB2D *ib, const float *f_in, float *f_out, void *wa, void *wb, int n,
+            void *stream);
+
+constexpr int kMaxIter = 4;
+
+__device__ __forceinline__ double block_sum_d(double v, double *red) {
+  v = wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int w = 0; w < (int)((blockDim.x + 63) >> 6); ++w) s += red[w];
+  return s;
+}
+
+// feq_i(rho, u) = rho w_i (1 + 3 c.u + 4.5 (c.u)^2 - 1.5 |u|^2)   (lbm/basic.py:114-135)
+__device__ __forceinline__ void equilibrium_raw(float rho, float ux, float uy, float (&feq)[9]) {
+  const float usq = ux * ux + uy * uy;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    const float cu = (float)D2Q9::cx[i] * ux + (float)D2Q9::cy[i] * uy;
+    feq[i] = rho * D2Q9::w[i] * (1.f + 3.f * cu + 4.5f * cu * cu - 1.5f * usq);
+  }
+}
+
+// d feq_i / d u_a at (rho, u): rho w_i (3 c_ia + 9 (c_i.u) c_ia - 3 u_a)
+__device__ __forceinline__ float dfeq_du(int i, int a, float rho, float cu, float ua) {
+  const float c = a == 0 ? (float)D2Q9::cx[i] : (float)D2Q9::cy[i];
+  return rho * D2Q9::w[i] * (3.f * c + 9.f * cu * c - 3.f * ua);
+}
+
+// ------------------------------------------------------------------ walls
+// One workgroup; y loop.  adj holds fbar_{t+1} and becomes sbar (adjoint of
+// the streamed field before the walls).  fn = f_{t+1}.
+__global__ __launch_bounds__(1024) void k_adj_walls(float *adj, const float *__restrict__ fn, int nx, int ny,
+                                                    float u_in, double *u_in_part) {
+  __shared__ double red[16];
+  const size_t n = (size_t)nx * ny;
+  double part = 0.0;
+  for (int y = threadIdx.x; y < ny; y += blockDim.x) {
+    // left, equilibrium at (1, (u_in, 0)): the wall cell forgets s; d/du_in
+    {
+      const size_t c0 = (size_t)y;
+      float s = 0.f;
+#pragma unroll
+      for (int i = 0; i < 9; ++i) {
+        const float q = adj[i * n + c0];
+        s += q * dfeq_du(i, 0, 1.f, (float)D2Q9::cx[i] * u_in, u_in);
+        adj[i * n + c0] = 0.f;
+      }
+      part += (double)s;
+    }
+    // right, pressure NEE with rho_w = 1
+    {
+      const size_t cw = (size_t)(nx - 1) * ny + y, cb = (size_t)(nx - 2) * ny + y;
+      float q[9], fw[9], sb[9];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) {
+        q[i] = adj[i * n + cw];
+        fw[i] = fn[i * n + cw];
+        sb[i] = fn[i * n + cb];
+      }
+      // wall velocity: (u_n, u_t) = momentum of the written wall cell (its density is 1)
+      float un = 0.f, ut = 0.f, rb = 0.f, jbx = 0.f, jby = 0.f;
+#pragma unroll
+      for (int i = 0; i < 9; ++i) {
+        un += (float)D2Q9::cx[i] * fw[i];
+        ut += (float)D2Q9::cy[i] * fw[i];
+        rb += sb[i];
+        jbx += (float)D2Q9::cx[i] * sb[i];
+        jby += (float)D2Q9::cy[i] * sb[i];
+      }
+      const float ubx = jbx / rb, uby = jby / rb;
+      float gx = 0.f, gy = 0.f, rhobar = 0.f, uxbar = 0.f, uybar = 0.f;
+      float feqb[9];
+      equilibrium_raw(rb, ubx, uby, feqb);
+#pragma unroll
+      for (int i = 0; i < 9; ++i) {
+        const float cuw = (float)D2Q9::cx[i] * un + (float)D2Q9::cy[i] * ut;
+        gx += q[i] * dfeq_du(i, 0, 1.f, cuw, un);
+        gy += q[i] * dfeq_du(i, 1, 1.f, cuw, ut);
+        const float cub = (float)D2Q9::cx[i] * ubx + (float)D2Q9::cy[i] * uby;
+        rhobar -= q[i] * feqb[i] / rb;
+        uxbar -= q[i] * dfeq_du(i, 0, rb, cub, ubx);
+        uybar -= q[i] * dfeq_du(i, 1, rb, cub, uby);
+      }
+#pragma unroll
+      for (int j = 0; j < 9; ++j) {
+        // d u_n / d s_j = -sign / rho_w * coef_j (right wall: sign = -1): 1 for c_x = 0, 2 for c_x > 0
+        const float coef = D2Q9::cx[j] == 0 ? 1.f : (D2Q9::cx[j] > 0 ? 2.f : 0.f);
+        adj[j * n + cw] = gx * coef;
+        const float cxj = (float)D2Q9::cx[j], cyj = (float)D2Q9::cy[j];
+        const float extra = q[j] + gy * (cyj - uby) / rb + rhobar + (uxbar * (cxj - ubx) + uybar * (cyj - uby)) / rb;
+        adj[j * n + cb] += extra;
+      }
+    }
+  }
+  const double tot = block_sum_d(part, red);
+  if (threadIdx.x == 0) *u_in_part = tot;
+}
+
+// ------------------------------------------------------------------ bulk
+struct AdjBulk {
+  int nx, ny, S;
+  float rates[9];
+  double win_x0, win_y0;
+};
+
+__global__ __launch_bounds__(256) void k_adj_bulk(AdjBulk P, const float *__restrict__ sbar, float *__restrict__ fbar,
+                                                  const float *__restrict__ ft, const float *__restrict__ gwin,
+                                                  const double *__restrict__ body, float *__restrict__ gbar) {
+  const int y = blockIdx.x * blockDim.x + threadIdx.x, x = blockIdx.y;
+  if (y >= P.ny) return;
+  const size_t n = (size_t)P.nx * P.ny, cell = (size_t)x * P.ny + y;
+  const int S = P.S;
+  const int xc = min(max((int)trunc(P.win_x0 + body[4]), 0), P.nx - S);
+  const int yc = min(max((int)trunc(P.win_y0 + body[5]), 0), P.ny - S);
+  const int r0 = x - xc, r1 = y - yc;
+  const bool in_win = r0 >= 0 && r0 < S && r1 >= 0 && r1 < S;
+  float pb[9], f[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    int xs = x + D2Q9::cx[i], ys = y + D2Q9::cy[i];
+    xs = xs < 0 ? xs + P.nx : (xs >= P.nx ? xs - P.nx : xs);
+    ys = ys < 0 ? ys + P.ny : (ys >= P.ny ? ys - P.ny : ys);
+    pb[i] = sbar[i * n + (size_t)xs * P.ny + ys];
+    f[i] = ft[i * n + cell];
+  }
+  float rho = 0.f, jx = 0.f, jy = 0.f;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    rho += f[i];
+    jx += (float)D2Q9::cx[i] * f[i];
+    jy += (float)D2Q9::cy[i] * f[i];
+  }
+  float gx = 0.f, gy = 0.f;
+  if (in_win) {
+    gx = gwin[r0 * S + r1];
+    gy = gwin[S * S + r0 * S + r1];
+  }
+  const float usx = jx / rho + gx * 0.5f / rho, usy = jy / rho + gy * 0.5f / rho;
+  // moments of pbar: mb_k = |row k|^-2 (M pbar)_k; q = A^T pbar, r = B^T pbar
+  float mb[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) mb[k] = mrow(k, pb) * Mrt2::inv_norm[k];
+  float q[9], r[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    float sq = 0.f, sr = 0.f;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const float m = (float)Mrt2::M[k][i];
+      sq += m * (P.rates[k] * mb[k]);
+      sr += m * ((1.f - 0.5f * P.rates[k]) * mb[k]);
+    }
+    q[i] = sq;
+    r[i] = sr;
+  }
+  float feq[9];
+  equilibrium_raw(rho, usx, usy, feq);
+  float rhobar = 0.f, usxb = 0.f, usyb = 0.f, gxb = 0.f, gyb = 0.f;
+  const float cg_x = gx, cg_y = gy;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    const float cx = (float)D2Q9::cx[i], cy = (float)D2Q9::cy[i];
+    const float cu = cx * usx + cy * usy, cgv = cx * cg_x + cy * cg_y;
+    rhobar += q[i] * feq[i] / rho;
+    usxb += q[i] * dfeq_du(i, 0, rho, cu, usx) + r[i] * D2Q9::w[i] * (-3.f * cg_x + 9.f * cx * cgv);
+    usyb += q[i] * dfeq_du(i, 1, rho, cu, usy) + r[i] * D2Q9::w[i] * (-3.f * cg_y + 9.f * cy * cgv);
+    gxb += r[i] * D2Q9::w[i] * (3.f * (cx - usx) + 9.f * cu * cx);
+    gyb += r[i] * D2Q9::w[i] * (3.f * (cy - usy) + 9.f * cu * cy);
+  }
+  gxb += usxb * 0.5f / rho;
+  gyb += usyb * 0.5f / rho;
+  const float jxb = usxb / rho, jyb = usyb / rho;
+  rhobar -= (usxb * usx + usyb * usy) / rho;
+#pragma unroll
+  for (int i = 0; i < 9; ++i)
+    fbar[i * n + cell] = pb[i] - q[i] + rhobar + jxb * (float)D2Q9::cx[i] + jyb * (float)D2Q9::cy[i];
+  if (in_win) {
+    gbar[r0 * S + r1] = gxb;
+    gbar[S * S + r0 * S + r1] = gyb;
+  }
+}
+
+// ------------------------------------------------------------------ IB workgroup
+struct AdjIB {
+  int nx, ny;
+  double inv_T;
+};
+
+template <int NIT>
+__global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const float *__restrict__ ft,
+                                                 const float *__restrict__ gbar, float *__restrict__ fbar,
+                                                 const double *__restrict__ body, const double *__restrict__ body1,
+                                                 double *__restrict__ adj_body, double *__restrict__ grad_delta,
+                                                 const double *__restrict__ dir_x, const double *__restrict__ dir_y) {
+  extern __shared__ float lds[];
+  const int S = ib.win, SS = S * S, M = ib.n_markers;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const size_t n = (size_t)P.nx * P.ny;
+  float *uw = lds;            // [2][SS] window velocity of f_t
+  float *G = lds + 2 * SS;    // [2][SS] spread grid
+  __shared__ double s_body[6], s_ab[6], s_misc[4];
+  __shared__ double red[16];
+  __shared__ int s_org[4];
+  if (tid < 6) {
+    s_body[tid] = body[tid];
+    s_ab[tid] = adj_body[tid];
+  }
+  __syncthreads();
+  if (tid == 0) {
+    // objective term and Newmark adjoint (dyn.py:5-51), per DOF
+    s_ab[5] += 2.0 * body1[5] * P.inv_T;
+    const double dt = ib.dt, c1 = ib.gamma * dt, c2 = ib.beta * dt * dt;
+    const double Mk = ib.mass + c1 * ib.damping + c2 * ib.stiffness;
+    double kbar = 0.0;
+    for (int k = 0; k < 2; ++k) {
+      const bool on = (ib.dof_mask >> k) & 1;
+      const double ab = on ? s_ab[k] : 0.0, vb = on ? s_ab[2 + k] : 0.0, db = on ? s_ab[4 + k] : 0.0;
+      const double a = s_body[k], v = s_body[2 + k], d = s_body[4 + k];
+      const double v2 = d + dt * v + dt * dt * (0.5 - ib.beta) * a;
+      const double an = body1[k];
+      const double anb = ab + c1 * vb + c2 * db;
+      const double Hb = anb / Mk;
+      kbar += anb * (-v2 - an * c2) / Mk;
+      const double v1b = vb - ib.damping * anb / Mk, v2b = db - ib.stiffness * anb / Mk;
+      s_ab[k] = dt * (1.0 - ib.gamma) * v1b + dt * dt * (0.5 - ib.beta) * v2b + Hb * ib.area;
+      s_ab[2 + k] = v1b + dt * v2b;
+      s_ab[4 + k] = v2b;
+      s_misc[k] = Hb;
+    }
+    s_misc[2] = kbar;
+    int o[4];
+    const int x0 = (int)trunc(ib.win_x0 + s_body[4]), y0 = (int)trunc(ib.win_y0 + s_body[5]);
+    o[0] = x0;
+    o[1] = y0;
+    o[2] = min(max(x0, 0), P.nx - S);
+    o[3] = min(max(y0, 0), P.ny - S);
+    for (int k = 0; k < 4; ++k) s_org[k] = o[k];
+  }
+  __syncthreads();
+  const int x0 = s_org[0], y0 = s_org[1], xc = s_org[2], yc = s_org[3];
+  // window velocity of f_t
+  for (int k = tid; k < SS; k += nt) {
+    const size_t cell = (size_t)(xc + k / S) * P.ny + (yc + k % S);
+    float rho = 0.f, jx = 0.f, jy = 0.f;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      const float v = ft[i * n + cell];
+      rho += v;
+      jx += (float)D2Q9::cx[i] * v;
+      jy += (float)D2Q9::cy[i] * v;
+    }
+    uw[k] = jx / rho;
+    uw[SS + k] = jy / rho;
+  }
+  // marker of this thread (lane-major, as the forward) and its stencil + derivatives
+  const int m = (tid & 63) * (nt >> 6) + (tid >> 6);
+  const bool act = m < M;
+  float wx[4] = {0, 0, 0, 0}, wy[4] = {0, 0, 0, 0}, dwx[4] = {0, 0, 0, 0}, dwy[4] = {0, 0, 0, 0};
+  int base = 0;
+  float ds2 = 0.f;
+  if (act) {
+    const float rx = (float)(ib.marker_x[m] + s_body[4] - (double)x0);
+    const float ry = (float)(ib.marker_y[m] + s_body[5] - (double)y0);
+    const int refx = (int)floorf(rx), refy = (int)floorf(ry);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float ax = (float)(refx - 1 + i) - rx, ay = (float)(refy - 1 + i) - ry;
+      wx[i] = delta_kernel(ib.kernel, ax);
+      wy[i] = delta_kernel(ib.kernel, ay);
+      dwx[i] = delta_kernel_deriv(ib.kernel, ax);
+      dwy[i] = delta_kernel_deriv(ib.kernel, ay);
+    }
+    base = (refx - 1) * S + (refy - 1);
+    ds2 = (ib.ds_arr ? ib.ds_arr[m] : ib.ds) * 2.0f;
+  }
+  __syncthreads();
+  const float Ux = (float)s_body[2], Uy = (float)s_body[3];
+  float rxb = 0.f, ryb = 0.f;   // adjoint of the marker position relative to the window
+  // gather of a 2-component grid at the stencil: value sums and W-bar terms with coefficients (cx, cy)
+  auto gather = [&](const float *grid, bool scatter_rule, float cx, float cy, float &vx, float &vy) {
+    vx = vy = 0.f;
+    if (!act) return;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int p = base + i * S + j;
+        int gi;
+        if (scatter_rule) {
+          gi = (int)scatter_index(p, SS);
+          if (gi < 0) continue;
+        } else {
+          gi = (int)gather_index(p, SS);
+        }
+        const float w = wx[i] * wy[j];
+        const float gxv = grid[gi], gyv = grid[SS + gi];
+        vx += w * gxv;
+        vy += w * gyv;
+        const float wb = cx * gxv + cy * gyv;   // W-bar of this stencil point
+        rxb -= wb * dwx[i] * wy[j];
+        ryb -= wb * wx[i] * dwy[j];
+      }
+  };
+  // spread of (vx, vy) into G with the given index rule
+  auto spread = [&](bool scatter_rule, float vx, float vy) {
+    for (int k = tid; k < 2 * SS; k += nt) G[k] = 0.f;
+    __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int p = base + i * S + j;
+          const int gi = scatter_rule ? (int)scatter_index(p, SS) : (int)gather_index(p, SS);
+          if (gi >= 0) {
+            const float w = wx[i] * wy[j];
+            atomicAdd(&G[gi], vx * w);
+            atomicAdd(&G[SS + gi], vy * w);
+          }
+        }
+    }
+    __syncthreads();
+  };
+  // ---- forward replay of the MDF loop (ib/mdf.py:45-57)
+  float ubx, uby, dFx[NIT], dFy[NIT], Fx = 0.f, Fy = 0.f;
+  gather(uw, false, 0.f, 0.f, ubx, uby);   // no W-bar term yet (coefficients 0)
+#pragma unroll
+  for (int k = 0; k < NIT; ++k) {
+    dFx[k] = (Ux - ubx) * ds2;
+    dFy[k] = (Uy - uby) * ds2;
+    Fx += dFx[k];
+    Fy += dFy[k];
+    spread(true, dFx[k], dFy[k]);
+    float ax, ay;
+    if (act) {
+      ax = ay = 0.f;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int gi = (int)gather_index(base + i * S + j, SS);
+          const float w = wx[i] * wy[j];
+          ax += w * (G[gi] * 0.5f);
+          ay += w * (G[SS + gi] * 0.5f);
+        }
+      ubx += ax;
+      uby += ay;
+    }
+    __syncthreads();
+  }
+  // ---- reverse
+  // g = W^T F (scatter rule); H = -sum F + a area
+  float Fbx, Fby;
+  gather(gbar, true, Fx, Fy, Fbx, Fby);
+  Fbx -= (float)s_misc[0];
+  Fby -= (float)s_misc[1];
+  float ubbx = 0.f, ubby = 0.f, Vbx = 0.f, Vby = 0.f;
+#pragma unroll
+  for (int k = NIT - 1; k >= 0; --k) {
+    // ub_{k+1} = ub_k + W (0.5 A_k), A_k = W^T dF_k
+    spread(true, dFx[k], dFy[k]);
+    {
+      float vx, vy;
+      gather(G, false, 0.5f * ubbx, 0.5f * ubby, vx, vy);
+    }
+    __syncthreads();
+    spread(false, 0.5f * ubbx, 0.5f * ubby);   // Abar_k = 0.5 W^T ubbar_{k+1} (gather rule)
+    float abx, aby;
+    gather(G, true, dFx[k], dFy[k], abx, aby);  // dFbar += W Abar_k (scatter rule)
+    __syncthreads();
+    const float dFbx = Fbx + abx, dFby = Fby + aby;
+    ubbx -= ds2 * dFbx;
+    ubby -= ds2 * dFby;
+    Vbx += ds2 * dFbx;
+    Vby += ds2 * dFby;
+  }
+  // ub_0 = W u: W-bar term and the window-velocity adjoint ubar = W^T ubbar_0 (gather rule)
+  {
+    float vx, vy;
+    gather(uw, false, ubbx, ubby, vx, vy);
+  }
+  spread(false, ubbx, ubby);
+  // fbar_t += (c_i - u) . ubar / rho on the window cells
+  for (int k = tid; k < SS; k += nt) {
+    const float ubx0 = G[k], uby0 = G[SS + k];
+    if (ubx0 == 0.f && uby0 == 0.f) continue;
+    const size_t cell = (size_t)(xc + k / S) * P.ny + (yc + k % S);
+    float rho = 0.f;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) rho += ft[i * n + cell];
+    const float ux = uw[k], uy = uw[SS + k];
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+      fbar[i * n + cell] += (ubx0 * ((float)D2Q9::cx[i] - ux) + uby0 * ((float)D2Q9::cy[i] - uy)) / rho;
+  }
+  // marker-position adjoints: delta (radial direction), displacement
+  if (act) grad_delta[m] += (double)rxb * dir_x[m] + (double)ryb * dir_y[m];
+  const double srx = block_sum_d(act ? (double)rxb : 0.0, red);
+  const double sry = block_sum_d(act ? (double)ryb : 0.0, red);
+  const double svx = block_sum_d(act ? (double)Vbx : 0.0, red);
+  const double svy = block_sum_d(act ? (double)Vby : 0.0, red);
+  if (tid == 0) {
+    adj_body[0] = s_ab[0];
+    adj_body[1] = s_ab[1];
+    adj_body[2] = s_ab[2] + svx;
+    adj_body[3] = s_ab[3] + svy;
+    adj_body[4] = s_ab[4] + srx;
+    adj_body[5] = s_ab[5] + sry;
+    adj_body[6] += s_misc[2];
+  }
+}
+
+__global__ void k_adj_final(const double *hist_d, int T, const double *adj_body, const double *u_in_part, double *out) {
+  __shared__ double red[16];
+  double j = 0.0, u = 0.0;
+  for (int t = threadIdx.x; t < T; t += blockDim.x) {
+    j += hist_d[2 * t + 1] * hist_d[2 * t + 1];
+    u += u_in_part[t];
+  }
+  j = block_sum_d(j, red);
+  u = block_sum_d(u, red);
+  if (threadIdx.x == 0) {
+    out[0] = j / T;
+    out[1] = adj_body[6];
+    out[2] = u;
+  }
+}
+
+// ------------------------------------------------------------------ driver
+
+int validate_adj(const VsxLbm *p, const VsxIB2D *ib, const VsxAdj2D *a) {
+  VSX_REQUIRE(a, "null adjoint parameters");
+  VSX_REQUIRE(p->collision == VSX_COLL_MRT && p->forcing == VSX_FORCE_GUO && p->storage == VSX_STORE_F32,
+              "the adjoint supports MRT collision with Guo forcing in FP32 storage");
+  VSX_REQUIRE(p->n_walls == 2 && p->walls[0].kind == VSX_WALL_EQ && p->walls[0].mode == VSX_MODE_PLAIN &&
+                  p->walls[0].loc == VSX_LOC_LEFT && !p->walls[0].ux_arr && !p->walls[0].uy_arr &&
+                  !p->walls[0].rho_arr && p->walls[0].rho == 1.f && p->walls[0].uy == 0.f &&
+                  p->walls[1].kind == VSX_WALL_NEE && p->walls[1].mode == VSX_MODE_PRESSURE &&
+                  p->walls[1].loc == VSX_LOC_RIGHT && !p->walls[1].rho_arr && p->walls[1].rho == 1.f,
+              "the adjoint supports walls [equilibrium inlet (left, rho 1, uy 0), pressure NEE outlet (right, rho 1)]");
+  VSX_REQUIRE(!p->g && p->g_uniform[0] == 0.f && p->g_uniform[1] == 0.f, "no extra body force in the adjoint");
+  VSX_REQUIRE(ib->n_markers <= kIbThreads, "the adjoint takes at most %d markers", kIbThreads);
+  VSX_REQUIRE(ib->n_iter >= 1 && ib->n_iter <= kMaxIter, "the adjoint takes 1..%d MDF iterations", kMaxIter);
+  VSX_REQUIRE(a->n_steps >= 1 && a->seg >= 1, "bad step count / segment");
+  VSX_REQUIRE(a->dir_x && a->dir_y && a->ckpt && a->ckpt_body && a->tape_f && a->tape_g && a->tape_body && a->adj_a &&
+                  a->adj_b && a->grid && a->adj_body && a->hist_d && a->u_in_part && a->grad_delta && a->out,
+              "null adjoint buffer");
+  VSX_REQUIRE(p->nx >= 4, "grid too small");
+  return VSX_OK;
+}
+
+template <int NIT>
+void launch_adj_ib(const VsxIB2D &ib, const AdjIB &P, const float *ft, const float *gbar, float *fbar,
+                   const double *body, const double *body1, const VsxAdj2D &a, hipStream_t s) {
+  const size_t lds = sizeof(float) * 4 * (size_t)ib.win * ib.win;
+  hipLaunchKernelGGL((k_adj_ib<NIT>), dim3(1), dim3(kIbThreads), lds, s, ib, P, ft, gbar, fbar, body, body1, a.adj_body,
+                     a.grad_delta, a.dir_x, a.dir_y);
+}
+
+int viv_adjoint(const VsxLbm *p, const VsxIB2D *ib, const VsxAdj2D *a, const float *f0, void *wa, void *wb,
+                float *f_tmp, void *stream) {
+  int rc = validate_lbm<D2Q9>(p);
+  if (rc) return rc;
+  rc = validate_ib2d(p, ib);
+  if (rc) return rc;
+  rc = validate_adj(p, ib, a);
+  if (rc) return rc;
+  VSX_REQUIRE(f0 && wa && wb && f_tmp, "null buffer");
+  int dev = 0, max_lds = 0;
+  VSX_HIP(hipGetDevice(&dev));
+  VSX_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  VSX_REQUIRE(sizeof(float) * 4 * (size_t)ib->win * ib->win + 1024 <= (size_t)max_lds,
+              "IB window %d too large for the adjoint workgroup", ib->win);
+  hipStream_t s = as_stream(stream);
+  const int T = a->n_steps, K = a->seg, nseg = (T + K - 1) / K;
+  // the fused forward pass waits on ready flags that must start below its first step
+  VSX_HIP(hipMemsetAsync(ib->win_origin, 0, sizeof(int32_t) * 8, s));
+  const size_t n = (size_t)p->nx * p->ny, fsz = 9 * n, gsz = 2 * (size_t)ib->win * ib->win;
+  VsxIB2D w = *ib;
+  // forward pass: checkpoints at segment starts
+  VSX_HIP(hipMemcpyAsync(a->ckpt, f0, sizeof(float) * fsz, hipMemcpyDeviceToDevice, s));
+  VSX_HIP(hipMemcpyAsync(a->ckpt_body, ib->body, sizeof(double) * 6, hipMemcpyDeviceToDevice, s));
+  for (int sg = 0; sg < nseg; ++sg) {
+    const int t0 = sg * K, nn = std::min(K, T - t0);
+    w.step0 = t0;
+    w.fused = 1;
+    w.hist_d = a->hist_d + 2 * (size_t)t0;
+    w.hist_h = nullptr;
+    float *dst = sg + 1 < nseg ? a->ckpt + (size_t)(sg + 1) * fsz : f_tmp;
+    rc = viv_run(p, &w, a->ckpt + (size_t)sg * fsz, dst, wa, wb, nn, stream);
+    if (rc) return rc;
+    if (sg + 1 < nseg)
+      VSX_HIP(hipMemcpyAsync(a->ckpt_body + 6 * (size_t)(sg + 1), ib->body + 6 * ((t0 + nn) & 1), sizeof(double) * 6,
+                             hipMemcpyDeviceToDevice, s));
+  }
+  // backward pass
+  VSX_HIP(hipMemsetAsync(a->adj_a, 0, sizeof(float) * fsz, s));
+  VSX_HIP(hipMemsetAsync(a->adj_body, 0, sizeof(double) * 8, s));
+  VSX_HIP(hipMemsetAsync(a->grad_delta, 0, sizeof(double) * ib->n_markers, s));
+  VSX_HIP(hipMemsetAsync(a->u_in_part, 0, sizeof(double) * T, s));
+  float *cur = a->adj_a, *nxt = a->adj_b;
+  AdjBulk PB;
+  PB.nx = p->nx;
+  PB.ny = p->ny;
+  PB.S = ib->win;
+  for (int k = 0; k < 9; ++k) PB.rates[k] = p->mrt_rates[k];
+  PB.win_x0 = ib->win_x0;
+  PB.win_y0 = ib->win_y0;
+  AdjIB PI;
+  PI.nx = p->nx;
+  PI.ny = p->ny;
+  PI.inv_T = 1.0 / T;
+  const dim3 bgrid((p->ny + 255) / 256, p->nx);
+  for (int sg = nseg - 1; sg >= 0; --sg) {
+    const int t0 = sg * K, nn = std::min(K, T - t0);
+    // recompute the segment one step at a time, keeping every state
+    VSX_HIP(hipMemcpyAsync(a->tape_f, a->ckpt + (size_t)sg * fsz, sizeof(float) * fsz, hipMemcpyDeviceToDevice, s));
+    VSX_HIP(hipMemcpyAsync(a->tape_body, a->ckpt_body + 6 * (size_t)sg, sizeof(double) * 6, hipMemcpyDeviceToDevice, s));
+    VSX_HIP(hipMemcpyAsync(ib->body + 6 * (t0 & 1), a->ckpt_body + 6 * (size_t)sg, sizeof(double) * 6,
+                           hipMemcpyDeviceToDevice, s));
+    for (int i = 0; i < nn; ++i) {
+      w.step0 = t0 + i;
+      w.fused = 0;
+      w.hist_d = nullptr;
+      rc = viv_run(p, &w, a->tape_f + (size_t)i * fsz, a->tape_f + (size_t)(i + 1) * fsz, wa, wb, 1, stream);
+      if (rc) return rc;
+      VSX_HIP(hipMemcpyAsync(a->tape_g + (size_t)i * gsz, ib->g_win, sizeof(float) * gsz, hipMemcpyDeviceToDevice, s));
+      VSX_HIP(hipMemcpyAsync(a->tape_body + 6 * (size_t)(i + 1), ib->body + 6 * ((t0 + i + 1) & 1), sizeof(double) * 6,
+                             hipMemcpyDeviceToDevice, s));
+    }
+    for (int i = nn - 1; i >= 0; --i) {
+      const int t = t0 + i;
+      const float *ft = a->tape_f + (size_t)i * fsz, *ft1 = a->tape_f + (size_t)(i + 1) * fsz;
+      const double *bt = a->tape_body + 6 * (size_t)i, *bt1 = a->tape_body + 6 * (size_t)(i + 1);
+      hipLaunchKernelGGL(k_adj_walls, dim3(1), dim3(1024), 0, s, cur, ft1, p->nx, p->ny, p->walls[0].ux,
+                         a->u_in_part + t);
+      hipLaunchKernelGGL(k_adj_bulk, bgrid, dim3(256), 0, s, PB, (const float *)cur, nxt, ft,
+                         (const float *)(a->tape_g + (size_t)i * gsz), bt, a->grid);
+      switch (ib->n_iter) {
+        case 1: launch_adj_ib<1>(*ib, PI, ft, a->grid, nxt, bt, bt1, *a, s); break;
+        case 2: launch_adj_ib<2>(*ib, PI, ft, a->grid, nxt, bt, bt1, *a, s); break;
+        case 3: launch_adj_ib<3>(*ib, PI, ft, a->grid, nxt, bt, bt1, *a, s); break;
+        default: launch_adj_ib<4>(*ib, PI, ft, a->grid, nxt, bt, bt1, *a, s); break;
+      }
+      VSX_CHECK_LAUNCH();
+      std::swap(cur, nxt);
+    }
+  }
+  hipLaunchKernelGGL(k_adj_final, dim3(1), dim3(1024), 0, s, (const double *)a->hist_d, T, (const double *)a->adj_body,
+                     (const double *)a->u_in_part, a->out);
+  VSX_CHECK_LAUNCH();
+  return VSX_OK;
+}
+
+}  // namespace vsx
+
+using namespace vsx;
+
+extern "C" int vsx_viv2d_adjoint(const VsxLbm *p, const VsxIB2D *ib, const VsxAdj2D *adj, const float *f0,
+                                 void *work_a, void *work_b, float *f_tmp, void *stream) {
+  return viv_adjoint(p, ib, adj, f0, work_a, work_b, f_tmp, stream);
+}

@@ -32,6 +32,35 @@ __device__ __forceinline__ float delta_kernel(int kind, float r) {
   }
 }
 
+// d phi / d r of the kernels above (the adjoint of the stencil weights with
+// respect to the marker position): d phi / d|r| times sign(r).
+__device__ __forceinline__ float delta_kernel_deriv(int kind, float r) {
+  const float a = fabsf(r);
+  const float sg = r > 0.f ? 1.f : (r < 0.f ? -1.f : 0.f);
+  float da;
+  switch (kind) {
+    case VSX_KERNEL_PESKIN_4PT:
+      if (a > 2.f) return 0.f;
+      if (a < 1.f) da = (-2.f + (4.f - 8.f * a) / (2.f * sqrtf(1.f + 4.f * a - 4.f * (a * a)))) * 0.125f;
+      else da = (-2.f - (12.f - 8.f * a) / (2.f * sqrtf(-7.f + 12.f * a - 4.f * (a * a)))) * 0.125f;
+      break;
+    case VSX_KERNEL_PESKIN_3PT:
+      if (a > 1.5f) return 0.f;
+      if (a < 0.5f) da = -a / sqrtf(1.f - 3.f * (a * a));
+      else da = (-3.f - (6.f - 6.f * a) / (2.f * sqrtf(-2.f + 6.f * a - 3.f * (a * a)))) / 6.f;
+      break;
+    case VSX_KERNEL_COSINE_4PT:
+      if (a > 2.f) return 0.f;
+      da = -0.125f * 3.14159265358979323846f * sinf(3.14159265358979323846f * a * 0.5f);
+      break;
+    default:
+      if (a > 1.f) return 0.f;
+      da = -1.f;
+      break;
+  }
+  return da * sg;
+}
+
 // gather index with JAX semantics (wrap negative once, clamp)
 __device__ __forceinline__ int64_t gather_index(int64_t i, int64_t n) {
   if (i < 0) i += n;

@@ -537,3 +537,98 @@ class VIVBatch:
         L.check(L.lib().vsx_viv2d_batch_graph(ctypes.byref(self.batch), L.ptr(f_in), L.ptr(f_out), L.ptr(wa),
                                               L.ptr(wb), int(n_steps), ctypes.byref(h)))
         return BatchGraph(h, (f_in, f_out, self))
+
+
+# ====================================================================== adjoint (2D VIV)
+
+class VIVAdjoint:
+    """Gradient of J = (1/T) sum_{t=1..T} d_y(t)^2 over a ``VIVCase`` run with
+    respect to the marker radial offsets delta [M], the spring stiffness and the
+    inlet velocity (``vsx_viv2d_adjoint``: hand-written reverse pass with
+    sqrt(T) checkpointing).  The case must be MRT + Guo with the equilibrium
+    inlet and the pressure-NEE outlet; the initial field is the case's seeded
+    inflow at the case's U0, the body starts at rest with velocity v0."""
+
+    def __init__(self, case: VIVCase, n_steps, seg=None, v0=(0.0, 0.0)):
+        require(case.collision == "mrt" and case.forcing == "guo", "the adjoint needs MRT collision with Guo forcing")
+        require(case.inlet == "eq" and case.outlet == "pressure_nee", "the adjoint needs the eq inlet and NEE outlet")
+        from . import lbm
+        dev = L.device()
+        self.case, self.T = case, int(n_steps)
+        self.seg = int(seg) if seg else max(1, int(round(math.sqrt(self.T))))
+        self.v0 = tuple(float(x) for x in v0)
+        nseg = (self.T + self.seg - 1) // self.seg
+        n = case.nx * case.ny
+        S = case.ib_size
+        M = case.n_markers
+        u = torch.as_tensor(case.initial_u(), dtype=torch.float32, device=dev)
+        self.f0 = lbm.get_equilibrium(torch.ones((case.nx, case.ny), device=dev), u).contiguous()
+        th = 2 * np.pi * np.arange(M) / M
+        self.dir_x = torch.as_tensor(np.cos(th), dtype=torch.float64, device=dev)
+        self.dir_y = torch.as_tensor(np.sin(th), dtype=torch.float64, device=dev)
+        z32 = lambda *s: torch.zeros(s, dtype=torch.float32, device=dev)
+        z64 = lambda *s: torch.zeros(s, dtype=torch.float64, device=dev)
+        self.buf = dict(ckpt=z32(nseg, 9 * n), ckpt_body=z64(nseg, 6), tape_f=z32(self.seg + 1, 9 * n),
+                        tape_g=z32(self.seg, 2 * S * S), tape_body=z64(self.seg + 1, 6), adj_a=z32(9 * n),
+                        adj_b=z32(9 * n), grid=z32(2 * S * S), adj_body=z64(8), hist_d=z64(self.T, 2),
+                        u_in_part=z64(self.T), grad_delta=z64(M), out=z64(3))
+        self.f_tmp = z32(9 * n)
+        a = L.VsxAdj2D()
+        a.n_steps, a.seg = self.T, self.seg
+        a.dir_x, a.dir_y = self.dir_x.data_ptr(), self.dir_y.data_ptr()
+        for k, t in self.buf.items():
+            setattr(a, k, t.data_ptr())
+        self.adj = a
+
+    def _setup(self, delta, stiffness, u_in):
+        c = self.case
+        lb = LBM((c.nx, c.ny), c.omega, collision="mrt", forcing="guo",
+                 walls=[dict(kind="eq", mode="plain", loc="left", ux_wall=float(u_in)),
+                        dict(kind="nee", mode="pressure", loc="right", rho_wall=1.0)])
+        body = _Body2D(c, L.device())
+        d = torch.as_tensor(delta, dtype=torch.float64, device=L.device()).reshape(-1)
+        require(d.numel() == c.n_markers, "delta must have one entry per marker")
+        body.marker_x.add_(d * self.dir_x)
+        body.marker_y.add_(d * self.dir_y)
+        body.ib.stiffness = float(stiffness)
+        body.set((0.0, 0.0), self.v0, (0.0, 0.0))
+        return lb, body
+
+    def evaluate(self, delta, stiffness, u_in):
+        """(J, dJ/ddelta [M] float64, dJ/dK, dJ/du_in)."""
+        lb, body = self._setup(delta, stiffness, u_in)
+        wa, wb = lb.work()
+        L.call("vsx_viv2d_adjoint", ctypes.byref(lb.p), ctypes.byref(body.ib), ctypes.byref(self.adj),
+               L.ptr(self.f0), L.ptr(wa), L.ptr(wb), L.ptr(self.f_tmp))
+        out = self.buf["out"].cpu().numpy()
+        return float(out[0]), self.buf["grad_delta"].clone(), float(out[1]), float(out[2])
+
+    def objective(self, delta, stiffness, u_in):
+        """J alone, from a plain fused run (no reverse pass)."""
+        lb, body = self._setup(delta, stiffness, u_in)
+        hd = torch.empty((self.T, 2), dtype=torch.float64, device=L.device())
+        body.ib.hist_d, body.ib.hist_h = hd.data_ptr(), None
+        body.ib.fused = 1
+        body.ib.step0 = 0
+        wa, wb = lb.work()
+        out = torch.empty_like(self.f0)
+        L.call("vsx_viv2d_run", ctypes.byref(lb.p), ctypes.byref(body.ib), L.ptr(self.f0), L.ptr(out), L.ptr(wa),
+               L.ptr(wb), self.T)
+        return float((hd[:, 1] ** 2).mean().item())
+
+
+class VIVObjective(torch.autograd.Function):
+    """J(delta, K, u_in) of a ``VIVAdjoint`` as an autograd function:
+    ``J = VIVObjective.apply(delta, K, u_in, adjoint)``."""
+
+    @staticmethod
+    def forward(ctx, delta, stiffness, u_in, adjoint):
+        J, gd, gk, gu = adjoint.evaluate(delta.detach(), float(stiffness), float(u_in))
+        ctx.save_for_backward(gd.to(delta.device, delta.dtype), torch.tensor(gk, dtype=torch.float64),
+                              torch.tensor(gu, dtype=torch.float64))
+        return torch.tensor(J, dtype=torch.float64)
+
+    @staticmethod
+    def backward(ctx, gJ):
+        gd, gk, gu = ctx.saved_tensors
+        return gJ * gd, (gJ * gk).to(gJ.dtype), (gJ * gu).to(gJ.dtype), None
