@@ -69,6 +69,86 @@ def run_gpu(args, storage, steps, warmup, fused=True):
                 lift_last=float(hh[-1, 1].item()), d_last=float(hd[-1, 1].item()))
 
 
+def timed(fn):
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = fn()
+    torch.cuda.synchronize()
+    return r, time.perf_counter() - t0
+
+
+def run_sphere(steps):
+    """3D flow past a sphere: D3Q19 BGK + EDM, 256x128x128, D = 24, 4096 Fibonacci markers, 2 MDF iterations."""
+    from vivsim_amd import sim
+    case = sim.SphereCase()
+    solver = sim.Sphere3D(case)
+    f = solver.initial_f()
+    f, _ = solver.run(f, 10, history=False)
+    (out, hh), sec = timed(lambda: solver.run(f, steps))
+    cells = case.nx * case.ny * case.nz
+    fx = float(hh[-1, 0].item())
+    return dict(glups=round(cells * steps / sec / 1e9, 3), ms_per_step=round(sec / steps * 1e3, 4), steps=steps,
+                cd_last=round(case.drag_coefficient(fx), 5), config="sphere3d_d3q19_bgk_edm_256x128x128_D24_m4096_mdf2")
+
+
+def sweep_cases(n=64, seed=0):
+    """The batched sweep: grids from {256x256, 512x256, 512x512, 1024x512} drawn with a seed, D = NY / 8,
+    markers round(pi D / 0.75), U_r ~ U[3, 8], Re ~ U[100, 1000], mass ratio ~ U[1, 10]; KBC + EDM, 3 MDF
+    iterations, cylinder at (NX / 4, NY / 2)."""
+    from vivsim_amd import sim
+    rng = np.random.default_rng(seed)
+    grids = [(256, 256), (512, 256), (512, 512), (1024, 512)]
+    out = []
+    for _ in range(n):
+        nx, ny = grids[rng.integers(len(grids))]
+        D = ny / 8
+        out.append(sim.VIVCase(nx=nx, ny=ny, D=D, cx=nx / 4, cy=ny / 2, n_markers=int(round(np.pi * D / 0.75)),
+                               Ur=float(rng.uniform(3, 8)), Re=float(rng.uniform(100, 1000)),
+                               mass_ratio=float(rng.uniform(1, 10)), collision="kbc", forcing="edm", n_iter=3))
+    return out
+
+
+def run_batch(steps, chunk=100):
+    import torch
+    from vivsim_amd import sim, tune
+    cases = sweep_cases()
+    b = sim.VIVBatch(cases, tile_y="auto", hist_cap=steps + 2 * chunk)
+    b.set_bodies(v=(0.0, 5e-5))
+    fa = b.initial_f()
+    fb = torch.empty_like(fa)
+    g_ab, g_ba = b.graph(fa, fb, chunk), b.graph(fb, fa, chunk)
+    g_ab.launch()
+    g_ba.launch()
+    reps = max(1, steps // (2 * chunk))
+
+    def go():
+        for _ in range(reps):
+            g_ab.launch()
+            g_ba.launch()
+
+    _, sec = timed(go)
+    b.check()
+    n = 2 * chunk * reps
+    return dict(glups=round(b.cells * n / sec / 1e9, 3), ms_per_step=round(sec / n * 1e3, 4), steps=n, cases=len(cases),
+                cells_per_step=b.cells, graph_steps=chunk, tile_y=sorted(set(b.tile_y)),
+                config="viv2d_batch64_kbc_edm_mdf3_hipgraph")
+
+
+def run_adjoint(steps):
+    """Adjoint / forward wall-time ratio: 512x256, D = 32 at (128, 128), 256 markers, MRT + Guo, Re = 150."""
+    from vivsim_amd import sim
+    case = sim.VIVCase(nx=512, ny=256, D=32.0, cx=128.0, cy=128.0, n_markers=256, Re=150.0, Ur=5.0, mass_ratio=2.0)
+    adj = sim.VIVAdjoint(case, steps, v0=(0.0, 1e-3 * case.U0))
+    delta = np.zeros(case.n_markers)
+    adj.objective(delta, case.stiffness, case.U0)           # warm-up
+    J_f, t_f = timed(lambda: adj.objective(delta, case.stiffness, case.U0))
+    (J, gd, gk, gu), t_a = timed(lambda: adj.evaluate(delta, case.stiffness, case.U0))
+    return dict(ratio=round(t_a / t_f, 3), forward_ms=round(t_f * 1e3, 2), adjoint_ms=round(t_a * 1e3, 2),
+                steps=steps, segment=adj.seg, J=J, dJ_dK=gk, dJ_du_in=gu,
+                dJ_ddelta_norm=float(gd.norm().item()), config="viv2d_adjoint_512x256_D32_m256_mrt_guo_re150")
+
+
 def run_cpu(args, seconds_budget):
     """Oracle VIV step (float32, one thread) on the same 1024^2 workload."""
     from threadpoolctl import threadpool_limits
@@ -104,6 +184,7 @@ def main():
     ap.add_argument("--cpu-max-steps", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--adjoint-steps", type=int, default=2000)
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -128,6 +209,15 @@ def main():
         r = run_gpu(args, args.storage, args.steps, args.warmup, fused=False)
         variants["separate_launches"] = dict(glups=round(r["glups"], 3), ms_per_step=round(r["ms_per_step"], 5),
                                              bulk_kernel_us=round(r["kernel_ms"] * 1e3, 3))
+        # beyond the 256 MB last-level cache: 4096^2 (2 x 1.2 GB populations), same body
+        big = argparse.Namespace(**{**vars(args), "nx": 4096, "ny": 4096})
+        r = run_gpu(big, "f32", 200, 20)
+        variants["grid_4096"] = dict(glups=round(r["glups"], 3), ms_per_step=round(r["ms_per_step"], 5),
+                                     kernel_us=round(r["kernel_ms"] * 1e3, 3), achieved_gbs=round(r["achieved_gbs"], 1),
+                                     frac=round(r["achieved_gbs"] / HBM_PEAK_GBS, 4))
+        variants["sphere3d"] = run_sphere(100)
+        variants["batch64"] = run_batch(400)
+        variants["adjoint"] = run_adjoint(args.adjoint_steps)
     cpu = None if args.no_cpu else run_cpu(args, args.cpu_seconds)
 
     line = {

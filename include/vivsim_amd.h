@@ -171,9 +171,10 @@ typedef struct VsxAdj2D {
   const double *dir_x, *dir_y;   /* device [M]: d X_m / d delta_m                                  */
   float *ckpt;              /* device [ceil(T/seg)][9][n]: post-streaming checkpoints              */
   double *ckpt_body;        /* device [ceil(T/seg)][6]                                              */
-  float *tape_f;            /* device [seg + 1][9][n]: post-streaming states of a segment           */
-  float *tape_g;            /* device [seg][2][S][S]: force windows of a segment                    */
-  double *tape_body;        /* device [seg + 1][6]: body states of a segment                        */
+  float *tape_f;            /* device [2][seg + 1][9][n]: post-streaming states of a segment (two
+                               segments in flight: recompute of one overlaps the reverse of the other) */
+  float *tape_g;            /* device [2][seg][2][S][S]: force windows of a segment                 */
+  double *tape_body;        /* device [2][seg + 1][6]: body states of a segment                     */
   float *adj_a, *adj_b;     /* device [9][n]: adjoint fields                                       */
   float *grid;              /* device [2][S][S]: adjoint of the force window                        */
   double *adj_body;         /* device [8]: adjoint of (a, v, d) and the stiffness accumulator        */

@@ -18,6 +18,7 @@
 //                interpolate / spread, kernel derivative phi' for the marker
 //                positions), window-velocity adjoint added to fbar_t
 #include <algorithm>
+#include <vector>
 
 #include "step_impl.hpp"
 #include "viv2d.hpp"
@@ -215,7 +216,9 @@ struct AdjIB {
   double inv_T;
 };
 
-template <int NIT>
+// KEEP: the forward replay's spread grids A_k stay in LDS for the reverse loop
+// (needs (4 + 2 NIT) S^2 floats of LDS), else they are spread again.
+template <int NIT, bool KEEP>
 __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const float *__restrict__ ft,
                                                  const float *__restrict__ gbar, float *__restrict__ fbar,
                                                  const double *__restrict__ body, const double *__restrict__ body1,
@@ -227,6 +230,7 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
   const size_t n = (size_t)P.nx * P.ny;
   float *uw = lds;            // [2][SS] window velocity of f_t
   float *G = lds + 2 * SS;    // [2][SS] spread grid
+  float *Ak = lds + 4 * SS;   // KEEP: [NIT][2][SS] forward spreads
   __shared__ double s_body[6], s_ab[6], s_misc[4];
   __shared__ double red[16];
   __shared__ int s_org[4];
@@ -330,9 +334,9 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
         ryb -= wb * wx[i] * dwy[j];
       }
   };
-  // spread of (vx, vy) into G with the given index rule
-  auto spread = [&](bool scatter_rule, float vx, float vy) {
-    for (int k = tid; k < 2 * SS; k += nt) G[k] = 0.f;
+  // spread of (vx, vy) into dst with the given index rule
+  auto spread_to = [&](float *dst, bool scatter_rule, float vx, float vy) {
+    for (int k = tid; k < 2 * SS; k += nt) dst[k] = 0.f;
     __syncthreads();
     if (act) {
 #pragma unroll
@@ -343,13 +347,14 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
           const int gi = scatter_rule ? (int)scatter_index(p, SS) : (int)gather_index(p, SS);
           if (gi >= 0) {
             const float w = wx[i] * wy[j];
-            atomicAdd(&G[gi], vx * w);
-            atomicAdd(&G[SS + gi], vy * w);
+            atomicAdd(&dst[gi], vx * w);
+            atomicAdd(&dst[SS + gi], vy * w);
           }
         }
     }
     __syncthreads();
   };
+  auto spread = [&](bool scatter_rule, float vx, float vy) { spread_to(G, scatter_rule, vx, vy); };
   // ---- forward replay of the MDF loop (ib/mdf.py:45-57)
   float ubx, uby, dFx[NIT], dFy[NIT], Fx = 0.f, Fy = 0.f;
   gather(uw, false, 0.f, 0.f, ubx, uby);   // no W-bar term yet (coefficients 0)
@@ -359,7 +364,8 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
     dFy[k] = (Uy - uby) * ds2;
     Fx += dFx[k];
     Fy += dFy[k];
-    spread(true, dFx[k], dFy[k]);
+    float *A = KEEP ? Ak + 2 * SS * k : G;
+    spread_to(A, true, dFx[k], dFy[k]);
     float ax, ay;
     if (act) {
       ax = ay = 0.f;
@@ -369,8 +375,8 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
         for (int j = 0; j < 4; ++j) {
           const int gi = (int)gather_index(base + i * S + j, SS);
           const float w = wx[i] * wy[j];
-          ax += w * (G[gi] * 0.5f);
-          ay += w * (G[SS + gi] * 0.5f);
+          ax += w * (A[gi] * 0.5f);
+          ay += w * (A[SS + gi] * 0.5f);
         }
       ubx += ax;
       uby += ay;
@@ -387,10 +393,12 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
 #pragma unroll
   for (int k = NIT - 1; k >= 0; --k) {
     // ub_{k+1} = ub_k + W (0.5 A_k), A_k = W^T dF_k
-    spread(true, dFx[k], dFy[k]);
+    const float *A = G;
+    if constexpr (KEEP) A = Ak + 2 * SS * k;
+    else spread(true, dFx[k], dFy[k]);
     {
       float vx, vy;
-      gather(G, false, 0.5f * ubbx, 0.5f * ubby, vx, vy);
+      gather(A, false, 0.5f * ubbx, 0.5f * ubby, vx, vy);
     }
     __syncthreads();
     spread(false, 0.5f * ubbx, 0.5f * ubby);   // Abar_k = 0.5 W^T ubbar_{k+1} (gather rule)
@@ -480,10 +488,15 @@ int validate_adj(const VsxLbm *p, const VsxIB2D *ib, const VsxAdj2D *a) {
 
 template <int NIT>
 void launch_adj_ib(const VsxIB2D &ib, const AdjIB &P, const float *ft, const float *gbar, float *fbar,
-                   const double *body, const double *body1, const VsxAdj2D &a, hipStream_t s) {
-  const size_t lds = sizeof(float) * 4 * (size_t)ib.win * ib.win;
-  hipLaunchKernelGGL((k_adj_ib<NIT>), dim3(1), dim3(kIbThreads), lds, s, ib, P, ft, gbar, fbar, body, body1, a.adj_body,
-                     a.grad_delta, a.dir_x, a.dir_y);
+                   const double *body, const double *body1, const VsxAdj2D &a, int max_lds, hipStream_t s) {
+  const size_t SS = (size_t)ib.win * ib.win;
+  const size_t keep = sizeof(float) * (4 + 2 * NIT) * SS;
+  if (keep + 2048 <= (size_t)max_lds)
+    hipLaunchKernelGGL((k_adj_ib<NIT, true>), dim3(1), dim3(kIbThreads), keep, s, ib, P, ft, gbar, fbar, body, body1,
+                       a.adj_body, a.grad_delta, a.dir_x, a.dir_y);
+  else
+    hipLaunchKernelGGL((k_adj_ib<NIT, false>), dim3(1), dim3(kIbThreads), sizeof(float) * 4 * SS, s, ib, P, ft, gbar,
+                       fbar, body, body1, a.adj_body, a.grad_delta, a.dir_x, a.dir_y);
 }
 
 int viv_adjoint(const VsxLbm *p, const VsxIB2D *ib, const VsxAdj2D *a, const float *f0, void *wa, void *wb,
@@ -513,6 +526,7 @@ int viv_adjoint(const VsxLbm *p, const VsxIB2D *ib, const VsxAdj2D *a, const flo
     const int t0 = sg * K, nn = std::min(K, T - t0);
     w.step0 = t0;
     w.fused = 1;
+    w.g_win = ib->g_win;
     w.hist_d = a->hist_d + 2 * (size_t)t0;
     w.hist_h = nullptr;
     float *dst = sg + 1 < nseg ? a->ckpt + (size_t)(sg + 1) * fsz : f_tmp;
@@ -540,41 +554,76 @@ int viv_adjoint(const VsxLbm *p, const VsxIB2D *ib, const VsxAdj2D *a, const flo
   PI.ny = p->ny;
   PI.inv_T = 1.0 / T;
   const dim3 bgrid((p->ny + 255) / 256, p->nx);
-  for (int sg = nseg - 1; sg >= 0; --sg) {
+  // The recompute of segment sg - 1 (second stream) overlaps the reverse pass of
+  // segment sg (caller's stream): both are chains of small latency-bound
+  // kernels.  Tapes are double-buffered by segment parity.
+  hipStream_t s2 = nullptr;
+  VSX_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+  std::vector<hipEvent_t> ev_rec(nseg, nullptr), ev_rev(nseg, nullptr);
+  hipEvent_t ev_fwd = nullptr;
+  auto cleanup = [&]() {
+    for (auto e : ev_rec) if (e) (void)hipEventDestroy(e);
+    for (auto e : ev_rev) if (e) (void)hipEventDestroy(e);
+    if (ev_fwd) (void)hipEventDestroy(ev_fwd);
+    (void)hipStreamDestroy(s2);
+  };
+  auto check = [&](hipError_t e) { return e == hipSuccess ? VSX_OK : fail(VSX_ERR_HIP, "adjoint: %s", hipGetErrorString(e)); };
+  auto tf = [&](int sg, int i) { return a->tape_f + ((size_t)(sg & 1) * (K + 1) + i) * fsz; };
+  auto tg = [&](int sg, int i) { return a->tape_g + ((size_t)(sg & 1) * K + i) * gsz; };
+  auto tb = [&](int sg, int i) { return a->tape_body + ((size_t)(sg & 1) * (K + 1) + i) * 6; };
+  auto recompute = [&](int sg) -> int {
     const int t0 = sg * K, nn = std::min(K, T - t0);
-    // recompute the segment one step at a time, keeping every state
-    VSX_HIP(hipMemcpyAsync(a->tape_f, a->ckpt + (size_t)sg * fsz, sizeof(float) * fsz, hipMemcpyDeviceToDevice, s));
-    VSX_HIP(hipMemcpyAsync(a->tape_body, a->ckpt_body + 6 * (size_t)sg, sizeof(double) * 6, hipMemcpyDeviceToDevice, s));
-    VSX_HIP(hipMemcpyAsync(ib->body + 6 * (t0 & 1), a->ckpt_body + 6 * (size_t)sg, sizeof(double) * 6,
-                           hipMemcpyDeviceToDevice, s));
-    for (int i = 0; i < nn; ++i) {
+    int r = check(hipMemcpyAsync(tf(sg, 0), a->ckpt + (size_t)sg * fsz, sizeof(float) * fsz, hipMemcpyDeviceToDevice, s2));
+    if (!r) r = check(hipMemcpyAsync(tb(sg, 0), a->ckpt_body + 6 * (size_t)sg, sizeof(double) * 6, hipMemcpyDeviceToDevice, s2));
+    if (!r) r = check(hipMemcpyAsync(ib->body + 6 * (t0 & 1), a->ckpt_body + 6 * (size_t)sg, sizeof(double) * 6,
+                                     hipMemcpyDeviceToDevice, s2));
+    for (int i = 0; i < nn && !r; ++i) {
       w.step0 = t0 + i;
       w.fused = 0;
       w.hist_d = nullptr;
-      rc = viv_run(p, &w, a->tape_f + (size_t)i * fsz, a->tape_f + (size_t)(i + 1) * fsz, wa, wb, 1, stream);
-      if (rc) return rc;
-      VSX_HIP(hipMemcpyAsync(a->tape_g + (size_t)i * gsz, ib->g_win, sizeof(float) * gsz, hipMemcpyDeviceToDevice, s));
-      VSX_HIP(hipMemcpyAsync(a->tape_body + 6 * (size_t)(i + 1), ib->body + 6 * ((t0 + i + 1) & 1), sizeof(double) * 6,
-                             hipMemcpyDeviceToDevice, s));
+      w.g_win = tg(sg, i);   // the IB workgroup writes this step's force window into the tape
+      r = viv_run(p, &w, tf(sg, i), tf(sg, i + 1), wa, wb, 1, s2);
+      if (!r) r = check(hipMemcpyAsync(tb(sg, i + 1), ib->body + 6 * ((t0 + i + 1) & 1), sizeof(double) * 6,
+                                       hipMemcpyDeviceToDevice, s2));
     }
-    for (int i = nn - 1; i >= 0; --i) {
+    if (!r) r = check(hipEventCreateWithFlags(&ev_rec[sg], hipEventDisableTiming));
+    if (!r) r = check(hipEventRecord(ev_rec[sg], s2));
+    return r;
+  };
+  rc = check(hipEventCreateWithFlags(&ev_fwd, hipEventDisableTiming));
+  if (!rc) rc = check(hipEventRecord(ev_fwd, s));
+  if (!rc) rc = check(hipStreamWaitEvent(s2, ev_fwd, 0));
+  if (!rc) rc = recompute(nseg - 1);
+  for (int sg = nseg - 1; sg >= 0 && !rc; --sg) {
+    const int t0 = sg * K, nn = std::min(K, T - t0);
+    rc = check(hipStreamWaitEvent(s, ev_rec[sg], 0));
+    for (int i = nn - 1; i >= 0 && !rc; --i) {
       const int t = t0 + i;
-      const float *ft = a->tape_f + (size_t)i * fsz, *ft1 = a->tape_f + (size_t)(i + 1) * fsz;
-      const double *bt = a->tape_body + 6 * (size_t)i, *bt1 = a->tape_body + 6 * (size_t)(i + 1);
+      const float *ft = tf(sg, i), *ft1 = tf(sg, i + 1);
+      const double *bt = tb(sg, i), *bt1 = tb(sg, i + 1);
       hipLaunchKernelGGL(k_adj_walls, dim3(1), dim3(1024), 0, s, cur, ft1, p->nx, p->ny, p->walls[0].ux,
                          a->u_in_part + t);
       hipLaunchKernelGGL(k_adj_bulk, bgrid, dim3(256), 0, s, PB, (const float *)cur, nxt, ft,
-                         (const float *)(a->tape_g + (size_t)i * gsz), bt, a->grid);
+                         (const float *)tg(sg, i), bt, a->grid);
       switch (ib->n_iter) {
-        case 1: launch_adj_ib<1>(*ib, PI, ft, a->grid, nxt, bt, bt1, *a, s); break;
-        case 2: launch_adj_ib<2>(*ib, PI, ft, a->grid, nxt, bt, bt1, *a, s); break;
-        case 3: launch_adj_ib<3>(*ib, PI, ft, a->grid, nxt, bt, bt1, *a, s); break;
-        default: launch_adj_ib<4>(*ib, PI, ft, a->grid, nxt, bt, bt1, *a, s); break;
+        case 1: launch_adj_ib<1>(*ib, PI, ft, a->grid, nxt, bt, bt1, *a, max_lds, s); break;
+        case 2: launch_adj_ib<2>(*ib, PI, ft, a->grid, nxt, bt, bt1, *a, max_lds, s); break;
+        case 3: launch_adj_ib<3>(*ib, PI, ft, a->grid, nxt, bt, bt1, *a, max_lds, s); break;
+        default: launch_adj_ib<4>(*ib, PI, ft, a->grid, nxt, bt, bt1, *a, max_lds, s); break;
       }
-      VSX_CHECK_LAUNCH();
+      rc = check(hipGetLastError());
       std::swap(cur, nxt);
     }
+    if (!rc) rc = check(hipEventCreateWithFlags(&ev_rev[sg], hipEventDisableTiming));
+    if (!rc) rc = check(hipEventRecord(ev_rev[sg], s));
+    if (!rc && sg > 0) {
+      // tape parity (sg - 1) & 1 was last read by the reverse pass of segment sg + 1
+      if (sg + 1 < nseg) rc = check(hipStreamWaitEvent(s2, ev_rev[sg + 1], 0));
+      if (!rc) rc = recompute(sg - 1);
+    }
   }
+  cleanup();
+  if (rc) return rc;
   hipLaunchKernelGGL(k_adj_final, dim3(1), dim3(1024), 0, s, (const double *)a->hist_d, T, (const double *)a->adj_body,
                      (const double *)a->u_in_part, a->out);
   VSX_CHECK_LAUNCH();

@@ -1,15 +1,16 @@
 // Batched 2D VIV sweep: every case of a batch advances in one launch per step.
 //
 // Launch layout (grid = n_cases + total tiles, 1024 threads per workgroup):
-//   blocks [0, n_cases)           the IB workgroup of case b (multi-direct
-//                                 forcing, FP64 body force, Newmark)
+//   blocks [0, n_cases)           the IB workgroup of case b (window velocity,
+//                                 multi-direct forcing, FP64 body force, Newmark)
 //   blocks [n_cases, ...)         bulk tiles, through a block map: first the
-//                                 tiles of every case inside the x-band of its
-//                                 IB window, then all other tiles.
-// A tile that intersects its case's force window publishes the window velocity
-// of its cells to that case's IB workgroup, waits (bounded) for the case's
-// force window and then collides; the band tiles come first so that the IB
-// workgroups get their input early and run while the other tiles stream.
+//                                 tiles of every case outside the x-band of its
+//                                 IB window, then the tiles inside the bands.
+// A tile that intersects its case's force window waits (bounded) for the
+// case's ready flag; the band tiles come last so that by the time they start
+// the IB workgroups, which start first and need nothing from the tiles, are
+// normally done: with many cases the IB work of all of them overlaps the
+// streaming of the whole batch.
 // Every kernel reads the batch clock (global step index of the run's first
 // step) from device memory, so a captured HIP graph replays correctly: the
 // ready flags, tile counters, body slots and history rows all follow the
@@ -65,10 +66,9 @@ k_viv_batch(const BatchCase *__restrict__ tab, const uint32_t *__restrict__ bmap
   }
   if (b < n_cases) {
     if constexpr (IB) {
-      ib_block<SRC, PHASE == VSX_PHASE_FULL, 1, 2, 2, true>(c, bc.ib, src + bc.f_off,
-                                                            gstep < bc.hist_cap ? gstep : -1, gstep,
-                                                            (wx1 - wx0 + 1) * (wy1 - wy0 + 1), bc.ib.body + 6 * slot,
-                                                            bc.ib.body + 6 * (slot ^ 1), lds);
+      ib_block<SRC, PHASE == VSX_PHASE_FULL, 1, 2, 2, false>(c, bc.ib, src + bc.f_off,
+                                                             gstep < bc.hist_cap ? gstep : -1, gstep, 0,
+                                                             bc.ib.body + 6 * slot, bc.ib.body + 6 * (slot ^ 1), lds);
       publish_ready(bc.ib, gstep + 1);
     }
     return;
@@ -94,10 +94,7 @@ k_viv_batch(const BatchCase *__restrict__ tab, const uint32_t *__restrict__ bmap
     }
   }
   if constexpr (IB) {
-    if (win) {
-      publish_window_u(bc.ib, gstep, valid, co, org, f);
-      wait_ready(bc.ib, gstep + 1);
-    }
+    if (win) wait_ready(bc.ib, gstep + 1);
   }
   if (!valid) return;
   if constexpr (PHASE != VSX_PHASE_STREAM) finish<D2Q9, COLL>(c, cell, co, org, win, f);
@@ -214,8 +211,8 @@ int batch_prepare(const VsxVivCase *cases, int n, void *table, int32_t *clock, V
   std::vector<char> host(sizeof(BatchCase) * n + 4 * (outer.size() + band.size()));
   std::memcpy(host.data(), tab.data(), sizeof(BatchCase) * n);
   char *bm = host.data() + sizeof(BatchCase) * n;
-  std::memcpy(bm, band.data(), 4 * band.size());
-  std::memcpy(bm + 4 * band.size(), outer.data(), 4 * outer.size());
+  std::memcpy(bm, outer.data(), 4 * outer.size());
+  std::memcpy(bm + 4 * outer.size(), band.data(), 4 * band.size());
   VSX_HIP(hipMemcpy(table, host.data(), host.size(), hipMemcpyHostToDevice));
   out->n_cases = n;
   out->collision = p0.collision;

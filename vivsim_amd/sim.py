@@ -568,8 +568,8 @@ class VIVAdjoint:
         self.dir_y = torch.as_tensor(np.sin(th), dtype=torch.float64, device=dev)
         z32 = lambda *s: torch.zeros(s, dtype=torch.float32, device=dev)
         z64 = lambda *s: torch.zeros(s, dtype=torch.float64, device=dev)
-        self.buf = dict(ckpt=z32(nseg, 9 * n), ckpt_body=z64(nseg, 6), tape_f=z32(self.seg + 1, 9 * n),
-                        tape_g=z32(self.seg, 2 * S * S), tape_body=z64(self.seg + 1, 6), adj_a=z32(9 * n),
+        self.buf = dict(ckpt=z32(nseg, 9 * n), ckpt_body=z64(nseg, 6), tape_f=z32(2, self.seg + 1, 9 * n),
+                        tape_g=z32(2, self.seg, 2 * S * S), tape_body=z64(2, self.seg + 1, 6), adj_a=z32(9 * n),
                         adj_b=z32(9 * n), grid=z32(2 * S * S), adj_body=z64(8), hist_d=z64(self.T, 2),
                         u_in_part=z64(self.T), grad_delta=z64(M), out=z64(3))
         self.f_tmp = z32(9 * n)
