@@ -28,6 +28,10 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per k_viv_fused launch on the 1024^2 FP32 workload, from rocprofv3 FETCH_SIZE / WRITE_SIZE
+# passes scaled by a 256 MiB copy (profiles/k_viv_fused_viv2d_1024x1024_f32_pmc.md); a profiler run
+# cannot be part of the timed bench itself.
+TRAFFIC_1024_F32 = 79916654
 
 
 def kernel_times(lib, n):
@@ -185,7 +189,19 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=2000)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run 5 device-to-device copies of 256 MiB (calibration of HBM byte counters)")
     args = ap.parse_args()
+
+    if args.copy_probe:
+        import torch
+        a = torch.ones(64 * 2 ** 20, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4, "copies": 5}))
+        return 0
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -237,7 +253,8 @@ def main():
                    "nx": args.nx, "ny": args.ny, "markers": 512, "mdf_iterations": 3, "collision": "mrt",
                    "forcing": "guo", "kernel": "peskin_4pt", "storage": args.storage, "structure": "fp64 newmark"},
         "roofline": {"bound": "hbm", "achieved": round(main_run["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(main_run["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(main_run["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                     "traffic": TRAFFIC_1024_F32 if (args.nx, args.ny, args.storage) == (1024, 1024, "f32") else None,
                      "kernel": "k_viv_fused (IB workgroup + pull-stream/MRT/Guo tiles, one launch per step)",
                      "kernel_us": round(main_run["kernel_ms"] * 1e3, 3),
                      "bytes_per_launch": main_run["bytes_per_launch"], "launches": main_run["kernel_launches"]},
