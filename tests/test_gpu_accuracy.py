@@ -1,0 +1,60 @@
+"""Accuracy of the storage formats against the float64 oracle over a long run:
+the VIV case scaled to 256 x 256 (D = 20, 256 markers, MRT + Guo, Re = 100,
+U_r = 5, mass ratio 2), 1000 steps.  Errors are stated for the lift
+coefficient history, the transverse displacement history and the final
+velocity field, for FP32 and the 16-bit shifted storages (FP32 arithmetic).
+The measured values are written to gpurun_out/accuracy.json when that
+folder exists."""
+
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import lbm2d as L2
+from oracle import steps as S
+from tests.gpu_util import say, to_np
+
+pytestmark = pytest.mark.gpu
+
+GEOM = dict(nx=256, ny=256, D=20.0, cx=64.0, cy=128.0, n_markers=256, Re=100.0, Ur=5.0, mass_ratio=2.0, pad=8)
+STEPS = 1000
+V0 = np.array([0.0, 5e-4])
+
+
+@pytest.fixture(scope="module")
+def fp64():
+    c = S.VIV2D(**GEOM)
+    f, d, v, a, dh, hh = S.viv_run(c, c.initial_f(np.float64), STEPS, v=V0)
+    rho, u = L2.get_macroscopic(f)
+    return c, dh, hh, u
+
+
+def rel(got, ref):
+    return float(np.abs(got - ref).max() / max(np.abs(ref).max(), 1e-30))
+
+
+def test_storage_accuracy(gpu_lib, fp64):
+    from vivsim_amd import lbm, sim
+    c, dh, hh, u64 = fp64
+    cl = lambda h: h[:, 1] / (0.5 * c.U0 ** 2 * c.D)
+    out = {}
+    for storage in ("f32", "f16s", "bf16s"):
+        pc = sim.VIVCase(**GEOM)
+        solver = sim.VIV2D(pc, storage=storage)
+        solver.set_body(v=tuple(V0))
+        f0 = torch.as_tensor(c.initial_f(np.float64), dtype=torch.float32, device="cuda")
+        f, hd, hh_g = solver.run(f0, STEPS)
+        solver.check()
+        _, ug = lbm.get_macroscopic(f)
+        out[storage] = dict(lift_coefficient=rel(cl(to_np(hh_g)), cl(hh)), displacement=rel(to_np(hd)[:, 1], dh[:, 1]),
+                            velocity=rel(to_np(ug), u64))
+        say(f"{storage}: {out[storage]}")
+    out["case"] = dict(GEOM, steps=STEPS, collision="mrt", forcing="guo", reference="float64 NumPy oracle")
+    if os.path.isdir("gpurun_out"):
+        with open("gpurun_out/accuracy.json", "w") as fh:
+            json.dump(out, fh, indent=1)
+    assert out["f32"]["lift_coefficient"] < 1e-3 and out["f32"]["velocity"] < 1e-4
+    assert out["f16s"]["velocity"] < 2e-2 and out["bf16s"]["velocity"] < 2e-1
