@@ -66,7 +66,7 @@ k_viv_batch(const BatchCase *__restrict__ tab, const uint32_t *__restrict__ bmap
   }
   if (b < n_cases) {
     if constexpr (IB) {
-      ib_block<SRC, PHASE == VSX_PHASE_FULL, 1, 2, 2, false>(c, bc.ib, src + bc.f_off,
+      ib_block<SRC, PHASE == VSX_PHASE_FULL, 1, 2, 2, false, 1, false>(c, bc.ib, src + bc.f_off,
                                                              gstep < bc.hist_cap ? gstep : -1, gstep, 0,
                                                              bc.ib.body + 6 * slot, bc.ib.body + 6 * (slot ^ 1), lds);
       publish_ready(bc.ib, gstep + 1);

@@ -67,6 +67,16 @@ __device__ __forceinline__ int64_t gather_index(int64_t i, int64_t n) {
   return i < 0 ? 0 : (i >= n ? n - 1 : i);
 }
 
+// 32-bit forms for indices into an IB window (|i| < 2 n < 2^31)
+__device__ __forceinline__ int gather_index32(int i, int n) {
+  if (i < 0) i += n;
+  return i < 0 ? 0 : (i >= n ? n - 1 : i);
+}
+__device__ __forceinline__ int scatter_index32(int i, int n) {
+  if (i < 0) i += n;
+  return (i < 0 || i >= n) ? -1 : i;
+}
+
 // scatter index with JAX semantics (wrap negative once, drop out of range -> -1)
 __device__ __forceinline__ int64_t scatter_index(int64_t i, int64_t n) {
   if (i < 0) i += n;

@@ -11,10 +11,14 @@ __global__ __launch_bounds__(kIbThreads) void k_viv_ib(StepCtx c_arg, VsxIB2D ib
   const StepCtx &c = kernel_ctx();
   extern __shared__ float lds[];
   const int gstep = ib.step0 + step, slot = gstep & 1;
-  ib_block<SRC, PULL, MPT, 4, 0, false>(c, ib, src, step, gstep, 0, ib.body + 6 * slot, ib.body + 6 * (slot ^ 1),
+  ib_block<SRC, PULL, MPT, 4, 0, false, 4>(c, ib, src, step, gstep, 0, ib.body + 6 * slot, ib.body + 6 * (slot ^ 1),
                                         lds);
   publish_ready(ib, gstep + 1);
 }
+
+// Window velocity of the fused step: the window tiles publish the
+// post-streaming velocity of their cells (store_window_u, count_window_tile)
+// and the IB workgroup waits for all of them.
 
 template <class SRC, class DST, int COLL>
 __global__ __attribute__((amdgpu_flat_work_group_size(kTileCells, kTileCells), amdgpu_waves_per_eu(8, 8))) void
@@ -36,8 +40,8 @@ k_viv_fused(StepCtx c_arg, VsxIB2D ib, const typename SRC::T *__restrict__ src, 
   int wx0, wx1, wy0, wy1;
   window_tiles(T, org[0], org[1], ib.win, wx0, wx1, wy0, wy1);
   if (blockIdx.x == 0) {
-    ib_block<SRC, true, 1, 2, 1, true>(c, ib, src, step, gstep, (wx1 - wx0 + 1) * (wy1 - wy0 + 1), body_in,
-                                       ib.body + 6 * (slot ^ 1), lds);
+    ib_block<SRC, true, 1, 1, 1, true, 1>(c, ib, src, step, gstep, (wx1 - wx0 + 1) * (wy1 - wy0 + 1), body_in,
+                                          ib.body + 6 * (slot ^ 1), lds);
     publish_ready(ib, gstep + 1);
     return;
   }
@@ -56,7 +60,8 @@ k_viv_fused(StepCtx c_arg, VsxIB2D ib, const typename SRC::T *__restrict__ src, 
       if (cls >= 0) single_wall_state<D2Q9, SRC>(c, src, co, cls, f);
       else pull<D2Q9, SRC>(c, src, co[0], co[1], 0, f);
     }
-    publish_window_u(ib, gstep, valid, co, org, f);
+    store_window_u(ib, valid, co, org, f);
+    count_window_tile(ib, gstep);
     wait_ready(ib, gstep + 1);
     if (!valid) return;
     finish<D2Q9, COLL>(c, cell, co, org, true, f);

@@ -44,18 +44,41 @@ __device__ __noinline__ float2 window_wall_u(const StepCtx &c, const typename SR
 }
 
 // ------------------------------------------------------------------ the IB workgroup
-// LDS of the IB workgroup in 4-byte words (S = window side, M = markers):
-//   ug    [2][S*S]   window velocity, then the spread force
-//   with BINS (one marker per thread), from a 16-byte aligned offset:
-//   rec   [M][12]    base, -, wx0..3, wy0..3, fx, fy of the markers in bin order
-//                    (base: flat window index of stencil point (0, 0); fx, fy:
-//                    the force being spread)
-//   bptr  [S*S + 1]  markers binned by base (CSR: after placement bptr[b] = end of bin b)
-//   blist [M]        markers by bin (placement order)
-//   pos   [M]        bin-ordered position of each marker
-__host__ __device__ inline int ib_rec_offset(int S) { return (2 * S * S + 3) & ~3; }
+__host__ __device__ inline int imax(int a, int b) { return a > b ? a : b; }
+
+// LDS of the IB workgroup, offsets in 4-byte words (S = window side, SS = S * S,
+// M = markers):
+//   grid  float2 [SS]          window velocity, then the accumulated spread force
+//                              (x and y of a cell side by side); while the
+//                              segmented spread is built it holds the uint16
+//                              entry offsets of the active cells
+//   with BINS (one marker per thread), markers at their bin-order position p
+//   (markers whose stencil leaves the window at the end):
+//   rec   [max(10 M, SS + 1)]  16-byte aligned: wx [4][M], wy [4][M] stencil
+//                              weights, force [M] float2 (the force being
+//                              spread).  While the bins are built it holds the
+//                              int32 bin counters.
+//   bend  uint16 [SS + 1]      end of each bin in bin order
+//   list  [max(M, SS / 2)]     int32 markers by bin while the bins are built, then
+//                              the uint16 list of active window cells
+//   seg   (over bend and list once the segmented spread is built) first and
+//         last cell (uint16 [kIbThreads] each) and head partial sum (float2
+//         [kIbThreads]) of each thread's spread entries
+struct IbLds {
+  int rec, bend, list, seg, total;
+};
+__host__ __device__ inline IbLds ib_lds_layout(int S, int M) {
+  const int SS = S * S;
+  IbLds L;
+  L.rec = (2 * SS + 3) & ~3;
+  L.bend = (L.rec + imax(10 * M, SS + 1) + 3) & ~3;
+  L.list = L.bend + (SS + 2) / 2;
+  L.seg = L.bend;
+  L.total = L.bend + imax((SS + 2) / 2 + imax(M, (SS + 1) / 2), 3 * kIbThreads);
+  return L;
+}
 __host__ __device__ inline int ib_lds_words(int S, int M, bool bins) {
-  return bins ? ib_rec_offset(S) + 14 * M + S * S + 1 : 2 * S * S;
+  return bins ? ib_lds_layout(S, M).total : 2 * S * S;
 }
 
 // Exclusive prefix sum of a[0..n) in place by the whole workgroup.
@@ -91,42 +114,310 @@ __device__ __forceinline__ void block_exclusive_scan(int *a, int n, int *s_wsum)
   __syncthreads();
 }
 
-// Spread of the binned markers onto the window as a gather: window cell k
-// receives from the markers whose base lies in the 4 runs of 4 bins
-// k - a S - 3 .. k - a S (a = 0..3), in ascending bin and marker order, with
-// weight wx[a] * wy[k - a S - base].  Every cell is written exactly once.
-__device__ __forceinline__ void bin_spread(float *ug, const float *rec, const int *bptr, int S) {
-  const int SS = S * S;
-  for (int k = threadIdx.x; k < SS; k += blockDim.x) {
-    int e0[4], e1[4];
+// Bin run a of window cell k: the bins lo..lo+3 with lo = k - a S - 3 hold the
+// markers [e0, e0 + n) in bin order, and marker e lies in bin lo + j with
+// j = (e >= b1) + (e >= b2) + (e >= b3) (the ends of bins lo..lo+2; bins with a
+// negative index are empty).  Stencil point (a, 3 - j) of that marker is cell k.
+__device__ __forceinline__ void bin_run(const uint16_t *bend, int k, int a, int S, int &e0, int &n, int &b1, int &b2,
+                                        int &b3) {
+  const int hi = k - a * S, lo = hi - 3;
+  const int e1 = hi >= 0 ? bend[hi] : 0;
+  e0 = hi >= 0 && lo > 0 ? bend[lo - 1] : 0;
+  n = e1 - e0;
+  b1 = lo >= 0 ? bend[lo] : 0;
+  b2 = lo >= -1 ? bend[lo + 1] : 0;
+  b3 = lo >= -2 ? bend[lo + 2] : 0;
+}
+
+// Window cells that receive force from a binned marker (a non-empty bin run),
+// listed once per step; returns their number (the list order does not matter).
+__device__ __forceinline__ int active_cells(const uint16_t *bend, uint16_t *act, int S, int *s_count) {
+  const int SS = S * S, tid = threadIdx.x, lane = tid & 63;
+  if (tid == 0) *s_count = 0;
+  __syncthreads();
+  for (int k0 = 0; k0 < SS; k0 += blockDim.x) {
+    const int k = k0 + tid;
+    bool on = false;
+    if (k < SS) {
+      int tot = 0;
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      const int hi = k - a * S, lo = hi - 3;
-      e0[a] = hi >= 0 && lo > 0 ? bptr[lo - 1] : 0;
-      e1[a] = hi >= 0 ? bptr[hi] : 0;
+      for (int a = 0; a < 4; ++a) {
+        const int hi = k - a * S, lo = hi - 3;
+        tot += hi >= 0 ? bend[hi] - (lo > 0 ? bend[lo - 1] : 0) : 0;
+      }
+      on = tot != 0;
     }
-    float gx = 0.f, gy = 0.f;
+    const unsigned long long mask = __ballot(on);
+    int first = 0;
+    if (lane == 0 && mask) first = atomicAdd(s_count, (int)__popcll(mask));
+    first = __shfl(first, 0, 64);
+    if (on) act[first + __popcll(mask & ((1ull << lane) - 1ull))] = (uint16_t)k;
+  }
+  __syncthreads();
+  return *s_count;
+}
+
+// Spread of the binned markers' forces added onto the active window cells, as
+// a gather: cell k receives from the markers of its 4 bin runs (bin_run) with
+// weight wx[a] * wy[3 - j].  R of the runs are walked side by side (their
+// loads issued together), each summed in bin and marker order, and the run
+// sums are added in run order.  One thread per active cell: no atomics,
+// deterministic.
+template <int R>
+__device__ __forceinline__ void bin_spread(float2 *grid, const float *rec, int M, const uint16_t *bend,
+                                           const uint16_t *act, int n_act, int S) {
+  const float2 *force = reinterpret_cast<const float2 *>(rec + 8 * M);
+  for (int q = threadIdx.x; q < n_act; q += blockDim.x) {
+    const int k = act[q];
+    float gx[4], gy[4];
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      const int hi = k - a * S;
-#pragma unroll 2
-      for (int e = e0[a]; e < e1[a]; ++e) {
-        const float4 q0 = *reinterpret_cast<const float4 *>(rec + 12 * e);
-        const float4 q1 = *reinterpret_cast<const float4 *>(rec + 12 * e + 4);
-        const float4 q2 = *reinterpret_cast<const float4 *>(rec + 12 * e + 8);
-        const int b = hi - __float_as_int(q0.x);
-        const float wxa = a == 0 ? q0.z : a == 1 ? q0.w : a == 2 ? q1.x : q1.y;
-        float wyb = q1.z;   // flat selects, no branches
-        wyb = b == 1 ? q1.w : wyb;
-        wyb = b == 2 ? q2.x : wyb;
-        wyb = b == 3 ? q2.y : wyb;
-        const float w = wxa * wyb;
-        gx += q2.z * w;
-        gy += q2.w * w;
+    for (int h = 0; h < 4; h += R) {
+      int e0[R], n[R], b1[R], b2[R], b3[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        bin_run(bend, k, h + r, S, e0[r], n[r], b1[r], b2[r], b3[r]);
+        gx[h + r] = gy[h + r] = 0.f;
+      }
+      int len = n[0];
+#pragma unroll
+      for (int r = 1; r < R; ++r) len = max(len, n[r]);
+      for (int t = 0; t < len; ++t) {
+        // a finished run reads marker 0 (present whenever a run is non-empty)
+        // with weight 0
+        float w[R];
+        float2 fm[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int e = t < n[r] ? e0[r] + t : 0;
+          const int j = (e >= b1[r]) + (e >= b2[r]) + (e >= b3[r]);
+          w[r] = rec[(h + r) * M + e] * rec[(7 - j) * M + e];   // wx[a] * wy[3 - j]
+          w[r] = t < n[r] ? w[r] : 0.f;
+          fm[r] = force[e];
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          gx[h + r] += fm[r].x * w[r];
+          gy[h + r] += fm[r].y * w[r];
+        }
       }
     }
-    ug[k] = gx;
-    ug[SS + k] = gy;
+    float2 g = grid[k];
+    g.x += (gx[0] + gx[1]) + (gx[2] + gx[3]);
+    g.y += (gy[0] + gy[1]) + (gy[2] + gy[3]);
+    grid[k] = g;
+  }
+}
+
+// ------------------------------------------------------------------ segmented spread
+// The 16 stencil entries of every binned marker, ordered by window cell (cells
+// ascending; within a cell runs a = 0..3, markers in bin order, as bin_spread
+// sums them), kSpreadEPT consecutive entries per thread with their weights in
+// registers.  A spread is then kSpreadEPT force loads and FMAs per thread plus
+// a segmented sum: the thread holding the first entry of a cell owns it and
+// adds the partial sums of the following threads whose entries continue that
+// cell, in thread order.  Deterministic, no atomics, balanced over the threads.
+constexpr int kSpreadEPT = 8;
+
+struct SpreadSeg {
+  uint32_t ec[kSpreadEPT];   // marker position | cell << 16
+  float w[kSpreadEPT];       // stencil weight wx[a] * wy[b]
+  int nthr;                  // threads holding entries
+};
+
+__device__ __forceinline__ int seg_cell(uint32_t ec) { return (int)(ec >> 16); }
+
+// Entry count of window cell k (its 4 bin runs).
+__device__ __forceinline__ int cell_entries(const uint16_t *bend, int k, int S) {
+  int tot = 0;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int hi = k - a * S, lo = hi - 3;
+    tot += hi >= 0 ? bend[hi] - (lo > 0 ? bend[lo - 1] : 0) : 0;
+  }
+  return tot;
+}
+
+// Builds the entries (every thread calls it; ends with a barrier).  Returns
+// false when the binned markers have more entries than the workgroup holds
+// (16 n > kSpreadEPT nt); the caller then uses bin_spread.  act / rptr: uint16
+// scratch for the active cells and their first entry; fcell, lcell: first and
+// last cell of each thread's entries (written after the entries are built, so
+// they may overlay bend and act).
+__device__ __forceinline__ bool seg_build(const float *rec, int M, const uint16_t *bend, int S, uint16_t *act,
+                                          uint16_t *rptr, uint16_t *fcell, uint16_t *lcell, int *s_wsum, SpreadSeg &sg) {
+  const int SS = S * S, tid = threadIdx.x, nt = blockDim.x;
+  const int T = 16 * (int)bend[SS - 1];
+  sg.nthr = (T + kSpreadEPT - 1) / kSpreadEPT;
+  if (T > kSpreadEPT * nt) return false;   // uniform: every thread reads the same bend entry
+  // active cells in ascending order with their first entry: per-thread chunks
+  // of consecutive cells, one block scan of (entries << 16 | active cells)
+  const int chunk = (SS + nt - 1) / nt, k0 = min(tid * chunk, SS), k1 = min(k0 + chunk, SS);
+  int mine = 0;
+  for (int k = k0; k < k1; ++k) {
+    const int e = cell_entries(bend, k, S);
+    mine += e ? (e << 16) + 1 : 0;
+  }
+  int v = mine;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(v, o, 64);
+    if ((tid & 63) >= o) v += t;
+  }
+  if ((tid & 63) == 63) s_wsum[tid >> 6] = v;
+  __shared__ int s_nact;
+  __syncthreads();
+  if (tid == 0) {
+    int run = 0;
+    for (int w = 0; w < (nt + 63) / 64; ++w) {
+      const int x = s_wsum[w];
+      s_wsum[w] = run;
+      run += x;
+    }
+    s_nact = run & 0xffff;
+    rptr[s_nact] = (uint16_t)T;   // end of the last active cell
+  }
+  __syncthreads();
+  const int at = s_wsum[tid >> 6] + v - mine;
+  int q = at & 0xffff, r = at >> 16;
+  for (int k = k0; k < k1; ++k) {
+    const int e = cell_entries(bend, k, S);
+    if (e) {
+      act[q] = (uint16_t)k;
+      rptr[q] = (uint16_t)r;
+      ++q;
+      r += e;
+    }
+  }
+  __syncthreads();
+  // this thread's entries [g0, g0 + kSpreadEPT): the active cell holding g0 is
+  // the largest cq with rptr[cq] <= g0 (rptr ascending, rptr[n_act] = T)
+  const int g0 = tid * kSpreadEPT;
+  int cq = 0;
+  if (g0 < T) {
+    int l = 0, h = s_nact;
+    while (h - l > 1) {
+      const int m = (l + h) >> 1;
+      if ((int)rptr[m] <= g0) l = m;
+      else h = m;
+    }
+    cq = l;
+  }
+  // walk the entries: the cell of entry g is advanced when g reaches the next
+  // cell's first entry; the run of an entry and its marker are re-read from
+  // bend (once per step)
+  int k = 0, start = 0, next = 0x7fffffff;
+  if (g0 < T) {
+    k = act[cq];
+    start = rptr[cq];
+    next = rptr[cq + 1];
+  }
+#pragma unroll
+  for (int t = 0; t < kSpreadEPT; ++t) {
+    const int g = min(g0 + t, T - 1);   // past the end: the last entry again, with weight 0
+    if (g == next) {
+      ++cq;
+      k = act[cq];
+      start = next;
+      next = rptr[cq + 1];
+    }
+    int o = g - start, a = 0, e0 = 0, b1 = 0, b2 = 0, b3 = 0;
+#pragma unroll
+    for (int aa = 0; aa < 4; ++aa) {
+      int ea, na, c1, c2, c3;
+      bin_run(bend, k, aa, S, ea, na, c1, c2, c3);
+      const bool here = o >= 0 && o < na;
+      a = here ? aa : a;
+      e0 = here ? ea : e0;
+      b1 = here ? c1 : b1;
+      b2 = here ? c2 : b2;
+      b3 = here ? c3 : b3;
+      e0 = here ? e0 + o : e0;
+      o -= na;
+    }
+    const int e = e0;
+    const int j = (e >= b1) + (e >= b2) + (e >= b3);
+    sg.ec[t] = (uint32_t)e | ((uint32_t)k << 16);
+    sg.w[t] = g0 + t < T ? rec[a * M + e] * rec[(7 - j) * M + e] : 0.f;   // wx[a] * wy[3 - j]
+  }
+  __syncthreads();   // bend, act and rptr are dead from here
+  if (tid < sg.nthr) {
+    fcell[tid] = (uint16_t)seg_cell(sg.ec[0]);
+    lcell[tid] = (uint16_t)seg_cell(sg.ec[kSpreadEPT - 1]);
+  }
+  __syncthreads();
+  return true;
+}
+
+// One spread of the forces (rec force [M] float2) added onto the grid.
+__device__ __forceinline__ void seg_spread(float2 *grid, const float2 *force, const SpreadSeg &sg,
+                                           const uint16_t *fcell, const uint16_t *lcell, float2 *headp) {
+  const int tid = threadIdx.x;
+  int tail_cell = -1;
+  float2 tail = make_float2(0.f, 0.f);
+  if (tid < sg.nthr) {
+    const bool cont_prev = tid > 0 && fcell[tid] == lcell[tid - 1];
+    const bool cont_next = tid + 1 < sg.nthr && lcell[tid] == fcell[tid + 1];
+    // the entries are made opaque here: addresses derived from them and kept
+    // across the MDF loop would take registers of their own.  Forces are
+    // loaded half the entries at a time (register budget).
+    float sx = 0.f, sy = 0.f;
+    bool first = true;
+    int cur = seg_cell(sg.ec[0]);
+#pragma unroll
+    for (int h = 0; h < kSpreadEPT; h += kSpreadEPT / 2) {
+      uint32_t ec[kSpreadEPT / 2];
+      float2 fm[kSpreadEPT / 2];
+#pragma unroll
+      for (int t = 0; t < kSpreadEPT / 2; ++t) {
+        ec[t] = sg.ec[h + t];
+        asm volatile("" : "+v"(ec[t]));
+        fm[t] = force[ec[t] & 0xffffu];
+      }
+#pragma unroll
+      for (int t = 0; t < kSpreadEPT / 2; ++t) {
+        const int ct = seg_cell(ec[t]);
+        if (h + t > 0 && ct != cur) {   // the segment of cur ends inside this thread
+          if (first && cont_prev) {
+            headp[tid] = make_float2(sx, sy);
+          } else {
+            float2 g = grid[cur];
+            g.x += sx;
+            g.y += sy;
+            grid[cur] = g;
+          }
+          first = false;
+          sx = sy = 0.f;
+          cur = ct;
+        }
+        sx += sg.w[h + t] * fm[t].x;
+        sy += sg.w[h + t] * fm[t].y;
+      }
+      asm volatile("" : "+v"(sx), "+v"(sy));
+    }
+    if (first && cont_prev) {
+      headp[tid] = make_float2(sx, sy);
+    } else if (cont_next) {
+      tail_cell = cur;
+      tail = make_float2(sx, sy);
+    } else {
+      float2 g = grid[cur];
+      g.x += sx;
+      g.y += sy;
+      grid[cur] = g;
+    }
+  }
+  __syncthreads();
+  if (tail_cell >= 0) {
+    for (int j = tid + 1; j < sg.nthr && fcell[j] == tail_cell; ++j) {
+      const float2 h = headp[j];
+      tail.x += h.x;
+      tail.y += h.y;
+      if (lcell[j] != tail_cell) break;
+    }
+    float2 g = grid[tail_cell];
+    g.x += tail.x;
+    g.y += tail.y;
+    grid[tail_cell] = g;
   }
 }
 
@@ -143,41 +434,141 @@ __device__ __forceinline__ void spin_until(const VsxIB2D &ib, int32_t *flag, int
   }
 }
 
+// Interpolation of the window field at a marker (ib/stencil.py:54-78): 4 x 4
+// stencil from flat base index, gather rule for indices outside the window.
+__device__ __forceinline__ float2 interp_marker(const float2 *grid, int base, int S, const float (&wx)[4],
+                                                const float (&wy)[4]) {
+  const int SS = S * S;
+  // the indices are recomputed at every call: kept across the MDF loop they
+  // would take 16 registers
+  asm volatile("" : "+v"(base));
+  float sx = 0.f, sy = 0.f;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const float2 v = grid[gather_index32(base + i * S + j, SS)];
+      const float w = wx[i] * wy[j];
+      sx += w * v.x;
+      sy += w * v.y;
+    }
+    // one stencil row of loads in flight at a time (register budget)
+    asm volatile("" : "+v"(sx), "+v"(sy));
+  }
+  return make_float2(sx, sy);
+}
+
+// Spread of one marker's force with LDS atomics (scatter rule of
+// ib/stencil.py:106-110: indices outside the window are dropped).
+__device__ __forceinline__ void scatter_marker(float *gflat, int base, int S, const float (&wx)[4], const float (&wy)[4],
+                                               float fx, float fy) {
+  const int SS = S * S;
+  asm volatile("" : "+v"(base));   // indices recomputed at every call, as in interp_marker
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int si = scatter_index32(base + i * S + j, SS);
+      if (si >= 0) {
+        const float w = wx[i] * wy[j];
+        atomicAdd(&gflat[2 * si], fx * w);
+        atomicAdd(&gflat[2 * si + 1], fy * w);
+      }
+    }
+}
+
+// Window velocity by the IB workgroup, in rounds of UW cells per thread: the
+// loads of window cells base + j nt + tid (j < UW) off the domain walls ...
+template <class SRC, bool PULL, int UW>
+__device__ __forceinline__ void window_pull(const StepCtx &c, const typename SRC::T *__restrict__ src, int base, int S,
+                                            int xc, int yc, float (&f)[UW][9], bool (&on)[UW]) {
+  const int SS = S * S, nt = blockDim.x;
+#pragma unroll
+  for (int j = 0; j < UW; ++j) {
+    const int k = base + j * nt + (int)threadIdx.x;
+    const int co[3] = {xc + k / S, yc + k % S, 0};
+    on[j] = k < SS && !(PULL && c.p.n_walls > 0 && wall_class<D2Q9>(c, co) != -1);
+    if (on[j]) {
+      if constexpr (PULL) pull<D2Q9, SRC>(c, src, co[0], co[1], 0, f[j]);
+      else local<D2Q9, SRC>(c, src, cell_of<D2Q9>(c, co[0], co[1], 0), f[j]);
+    }
+  }
+}
+
+// ... and their macroscopic velocity into the grid.
+template <int UW>
+__device__ __forceinline__ void window_store(float2 *grid, int base, const float (&f)[UW][9], const bool (&on)[UW]) {
+#pragma unroll
+  for (int j = 0; j < UW; ++j) {
+    if (on[j]) {
+      float rho, u[3];
+      macroscopic<D2Q9>(f[j], rho, u);
+      grid[base + j * (int)blockDim.x + (int)threadIdx.x] = make_float2(u[0], u[1]);
+    }
+  }
+}
+
 // One workgroup per body (examples/2d/vortex_induced_vibration.py:99-137):
 //   window u     macroscopic velocity of the post-streaming state (pull + walls);
 //                U_GLOBAL: published in ib.g_win by the n_wtiles tiles that
 //                cover the window (counter ib.win_origin[4 + (gstep & 1)])
 //   stencil      ib/stencil.py:7-51 at marker coordinates (x_m + d) - origin (4x4)
-//   MDF          ib/mdf.py:35-64: interpolate, n_iter x (spread dF, interpolate
-//                0.5 * spread), final spread of the total force.  BINS: spread is
-//                a gather over the markers binned by stencil base cell (see
-//                bin_spread): no atomics, deterministic.
-//                Markers whose stencil leaves the window, and the path with
-//                several markers per thread, use LDS atomics with the scatter
-//                rule of ib/stencil.py:106-110
+//   MDF          ib/mdf.py:35-64: ub = interp(u); n_iter x (dF = (U - ub) 2 ds,
+//                F += dF, ub += interp(0.5 spread(dF))); force window spread(F).
+//                By linearity of spread and interp the spreads accumulate in one
+//                grid G = sum spread(dF), ub = ub0 + 0.5 interp(G), and the
+//                force window is the final G (n_iter spreads, no separate
+//                spread of F).  BINS: the spread is a gather over the markers
+//                binned by stencil base cell (bin_spread): no atomics,
+//                deterministic; the stencil weights live in LDS.  Markers whose
+//                stencil leaves the window, and the path with several markers
+//                per thread, use LDS atomics with the scatter rule
 //   body force   h = sum_m (-F_m) + a * area in FP64 (wave shuffles + LDS), dyn.py:126-136
 //   structure    Newmark-beta in FP64 on lane 0, dyn.py:5-51
 // Writes the force window to ib.g_win and the new body state to body_out.
-template <class SRC, bool PULL, int MPT, int UW, int TAG, bool U_GLOBAL>
+template <class SRC, bool PULL, int MPT, int UW, int TAG, bool U_GLOBAL, int SPREAD_RUNS = 2, bool SEG = true>
 __device__ __forceinline__ void ib_block(const StepCtx &c, const VsxIB2D &ib, const typename SRC::T *__restrict__ src,
                                          int hidx, int gstep, int n_wtiles, const double *body_in, double *body_out,
                                          float *lds) {
   const int S = ib.win, SS = S * S, M = ib.n_markers;
   constexpr bool BINS = MPT == 1;
-  float *ug = lds;
-  float *rec = lds + ib_rec_offset(S);
-  int *bptr = reinterpret_cast<int *>(rec + 12 * M);
-  int *blist = bptr + SS + 1;
-  int *pos = blist + M;
+  constexpr int WR = BINS ? 1 : MPT;   // markers whose weights stay in registers
+  float2 *grid = reinterpret_cast<float2 *>(lds);
+  float *gflat = lds;
+  const IbLds lay = ib_lds_layout(S, M);
+  float *rec = lds + lay.rec;
+  float2 *rforce = reinterpret_cast<float2 *>(rec + 8 * M);
+  int *cnt = reinterpret_cast<int *>(rec);
+  uint16_t *bend = reinterpret_cast<uint16_t *>(lds + lay.bend);
+  int *blist = reinterpret_cast<int *>(lds + lay.list);
+  uint16_t *act = reinterpret_cast<uint16_t *>(lds + lay.list);
   __shared__ double s_body[6];
   __shared__ int s_org[4];
-  __shared__ int s_irregular;
+  __shared__ int s_irregular, s_nact, s_nirr;
   __shared__ int s_wsum[kIbThreads / 64];
   __shared__ double s_red[2][kIbThreads / 64];
   const int tid = threadIdx.x, nt = blockDim.x;
 
+  // Marker q of this thread: consecutive lanes of a wavefront hold markers
+  // nt/64 apart along the body, so the lanes of one LDS access touch different
+  // cells.  The marker inputs are loaded first, beside the body state.
+  const int nwave = nt >> 6;
+  int mk[MPT];
+  double mx[MPT], my[MPT];
+  float ds2[MPT];
+#pragma unroll
+  for (int q = 0; q < MPT; ++q) {
+    mk[q] = q * nt + (tid & 63) * nwave + (tid >> 6);
+    mx[q] = my[q] = 0.0;
+    ds2[q] = 0.f;
+    if (mk[q] < M) {
+      mx[q] = ib.marker_x[mk[q]];
+      my[q] = ib.marker_y[mk[q]];
+      ds2[q] = (ib.ds_arr ? ib.ds_arr[mk[q]] : ib.ds) * 2.0f;
+    }
+  }
   if (tid < 6) s_body[tid] = body_in[tid];
-  if (tid == 0) s_irregular = 0;
+  if (tid == 0) s_irregular = s_nirr = 0;
   __syncthreads();
   if (tid == 0) {
     int o[4];
@@ -190,211 +581,191 @@ __device__ __forceinline__ void ib_block(const StepCtx &c, const VsxIB2D &ib, co
   __syncthreads();
   const int x0 = s_org[0], y0 = s_org[1], xc = s_org[2], yc = s_org[3];
 
-  // Markers first (they need no field data).  Marker q of this thread:
-  // consecutive lanes of a wavefront hold markers nt/64 apart along the body,
-  // so the lanes of one LDS access touch different cells.
-  const float Ux = (float)s_body[2], Uy = (float)s_body[3];
-  int mk[MPT], base_idx[MPT], bx[MPT], by[MPT];
+  // stencils of the markers (weights in registers until they are stored)
+  int base_idx[MPT];
   bool regular[MPT];
-  float wx[MPT][4], wy[MPT][4];
-  float ubx[MPT], uby[MPT], Fx[MPT], Fy[MPT], ds2[MPT];
-  const int nwave = nt >> 6;
+  float wx[WR][4], wy[WR][4];
+  float wx_t[4], wy_t[4];
 #pragma unroll
   for (int q = 0; q < MPT; ++q) {
-    mk[q] = q * nt + (tid & 63) * nwave + (tid >> 6);
-    ubx[q] = uby[q] = Fx[q] = Fy[q] = ds2[q] = 0.f;
-    base_idx[q] = bx[q] = by[q] = 0;
+    base_idx[q] = 0;
     regular[q] = false;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) wx[q][i] = wy[q][i] = 0.f;
+    for (int i = 0; i < 4; ++i) wx_t[i] = wy_t[i] = 0.f;
     if (mk[q] < M) {
-      const int m = mk[q];
-      const float rx = (float)(ib.marker_x[m] + s_body[4] - (double)x0);
-      const float ry = (float)(ib.marker_y[m] + s_body[5] - (double)y0);
+      const float rx = (float)(mx[q] + s_body[4] - (double)x0);
+      const float ry = (float)(my[q] + s_body[5] - (double)y0);
       const Axis<4> ax = axis_stencil<4>(ib.kernel, rx), ay = axis_stencil<4>(ib.kernel, ry);
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        wx[q][i] = ax.w[i];
-        wy[q][i] = ay.w[i];
+        wx_t[i] = ax.w[i];
+        wy_t[i] = ay.w[i];
       }
-      bx[q] = ax.ref - 1;   // window row / column of stencil point (0, 0)
-      by[q] = ay.ref - 1;
-      base_idx[q] = bx[q] * S + by[q];   // its flat window index
-      ds2[q] = (ib.ds_arr ? ib.ds_arr[m] : ib.ds) * 2.0f;
-      regular[q] = BINS && bx[q] >= 0 && bx[q] <= S - 4 && by[q] >= 0 && by[q] <= S - 4;
+      const int bx = ax.ref - 1, by = ay.ref - 1;   // window row / column of stencil point (0, 0)
+      base_idx[q] = bx * S + by;                    // its flat window index
+      regular[q] = BINS && bx >= 0 && bx <= S - 4 && by >= 0 && by <= S - 4;
       if (!regular[q]) s_irregular = 1;
     }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      wx[BINS ? 0 : q][i] = wx_t[i];
+      wy[BINS ? 0 : q][i] = wy_t[i];
+    }
   }
+
+  int my_pos = 0, n_act = 0;
+  bool use_seg = false;
+  SpreadSeg sg;
+  uint16_t *fcell = reinterpret_cast<uint16_t *>(lds + lay.seg), *lcell = fcell + kIbThreads;
+  float2 *headp = reinterpret_cast<float2 *>(lds + lay.seg + kIbThreads);
   if constexpr (BINS) {
     // regular markers binned by base cell: count, scan, place; a marker's
-    // position = start of its bin + its rank by marker index within the bin
-    for (int k = tid; k <= SS; k += nt) bptr[k] = 0;
+    // position = start of its bin + its rank by marker index within the bin;
+    // the other markers take the positions from M - 1 down
+    const bool live = mk[0] < M, binned = live && regular[0];
+    for (int k = tid; k <= SS; k += nt) cnt[k] = 0;
     __syncthreads();
-    if (mk[0] < M && regular[0]) atomicAdd(&bptr[base_idx[0]], 1);
+    if (binned) atomicAdd(&cnt[base_idx[0]], 1);
     __syncthreads();
-    block_exclusive_scan(bptr, SS, s_wsum);
-    if (mk[0] < M && regular[0]) blist[atomicAdd(&bptr[base_idx[0]], 1)] = mk[0];
+    block_exclusive_scan(cnt, SS, s_wsum);
+    if (binned) blist[atomicAdd(&cnt[base_idx[0]], 1)] = mk[0];
     __syncthreads();
-    if (mk[0] < M && regular[0]) {
-      const int s0 = base_idx[0] > 0 ? bptr[base_idx[0] - 1] : 0, e0 = bptr[base_idx[0]];
+    if (binned) {
+      const int s0 = base_idx[0] > 0 ? cnt[base_idx[0] - 1] : 0, e0 = cnt[base_idx[0]];
       int rank = 0;
       for (int i = s0; i < e0; ++i) rank += blist[i] < mk[0];
-      pos[mk[0]] = s0 + rank;
+      my_pos = s0 + rank;
+    } else if (live) {
+      my_pos = M - 1 - atomicAdd(&s_nirr, 1);
     }
-    __syncthreads();
-    if (mk[0] < M && regular[0]) {
-      const int e = pos[mk[0]];
-      float *r = rec + 12 * e;
-      r[0] = __int_as_float(base_idx[0]);
-      r[1] = 0.f;
+    for (int k = tid; k < SS; k += nt) bend[k] = (uint16_t)cnt[k];
+    __syncthreads();   // counters and marker list are dead from here
+    if (live) {
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        r[2 + i] = wx[0][i];
-        r[6 + i] = wy[0][i];
+        rec[i * M + my_pos] = wx[0][i];
+        rec[(4 + i) * M + my_pos] = wy[0][i];
       }
     }
+    if constexpr (SEG)
+      use_seg = seg_build(rec, M, bend, S, act, reinterpret_cast<uint16_t *>(lds), fcell, lcell, s_wsum, sg);
+    if (!use_seg) n_act = active_cells(bend, act, S, &s_nact);   // ends with a barrier
   }
 
   // window velocity
   if constexpr (U_GLOBAL) {
     if (tid == 0) {
-      int32_t *cnt = &ib.win_origin[4 + (gstep & 1)];
-      spin_until(ib, cnt, n_wtiles);
-      __hip_atomic_store(cnt, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);   // reused two steps later
+      int32_t *cnt_u = &ib.win_origin[4 + (gstep & 1)];
+      spin_until(ib, cnt_u, n_wtiles);
+      __hip_atomic_store(cnt_u, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);   // reused two steps later
     }
     __syncthreads();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    for (int k = tid; k < 2 * SS; k += nt) ug[k] = ib.g_win[k];
+    for (int k = tid; k < SS; k += nt) grid[k] = make_float2(ib.g_win[k], ib.g_win[SS + k]);
   } else {
     for (int base = 0; base < SS; base += UW * nt) {
-      float f[UW][9];
-      bool on_w[UW];
-#pragma unroll
-      for (int j = 0; j < UW; ++j) {
-        const int k = base + j * nt + tid;
-        on_w[j] = false;
-        if (k < SS) {
-          const int co[3] = {xc + k / S, yc + k % S, 0};
-          if constexpr (!PULL) {
-            local<D2Q9, SRC>(c, src, cell_of<D2Q9>(c, co[0], co[1], 0), f[j]);
-          } else {
-            on_w[j] = c.p.n_walls > 0 && wall_class<D2Q9>(c, co) != -1;
-            if (!on_w[j]) pull<D2Q9, SRC>(c, src, co[0], co[1], 0, f[j]);
-          }
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < UW; ++j) {
-        const int k = base + j * nt + tid;
-        if (k < SS) {
-          if (on_w[j]) {
-            const float2 uw = window_wall_u<SRC, TAG>(c, src, xc + k / S, yc + k % S);
-            ug[k] = uw.x;
-            ug[SS + k] = uw.y;
-          } else {
-            float rho, u[3];
-            macroscopic<D2Q9>(f[j], rho, u);
-            ug[k] = u[0];
-            ug[SS + k] = u[1];
-          }
-        }
+      float fu[UW][9];
+      bool fu_on[UW];
+      window_pull<SRC, PULL, UW>(c, src, base, S, xc, yc, fu, fu_on);
+      window_store<UW>(grid, base, fu, fu_on);
+    }
+    // cells on a domain wall (rare: only when the window touches a wall), one
+    // at a time, with nothing else in flight across the call
+    if (PULL && c.p.n_walls > 0) {
+      for (int k = tid; k < SS; k += nt) {
+        const int co[3] = {xc + k / S, yc + k % S, 0};
+        if (wall_class<D2Q9>(c, co) != -1) grid[k] = window_wall_u<SRC, TAG>(c, src, co[0], co[1]);
       }
     }
   }
   __syncthreads();
 
+  // weights of marker q (from LDS when binned)
+  auto weights = [&](int q, float(&wxq)[4], float(&wyq)[4]) {
+    if constexpr (BINS) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        wxq[i] = rec[i * M + my_pos];
+        wyq[i] = rec[(4 + i) * M + my_pos];
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        wxq[i] = wx[q][i];
+        wyq[i] = wy[q][i];
+      }
+    }
+  };
+
   // initial interpolation of the window velocity
+  float ub0x[MPT], ub0y[MPT], ubx[MPT], uby[MPT], Fx[MPT], Fy[MPT];
 #pragma unroll
   for (int q = 0; q < MPT; ++q) {
+    ub0x[q] = ub0y[q] = Fx[q] = Fy[q] = 0.f;
     if (mk[q] < M) {
-      float sx = 0.f, sy = 0.f;
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int gi = (int)gather_index(base_idx[q] + i * S + j, SS);
-          const float w = wx[q][i] * wy[q][j];
-          sx += w * ug[gi];
-          sy += w * ug[SS + gi];
-        }
-      ubx[q] = sx;
-      uby[q] = sy;
+      float wxq[4], wyq[4];
+      weights(q, wxq, wyq);
+      const float2 v = interp_marker(grid, base_idx[q], S, wxq, wyq);
+      ub0x[q] = v.x;
+      ub0y[q] = v.y;
     }
+    ubx[q] = ub0x[q];
+    uby[q] = ub0y[q];
   }
-  __syncthreads();   // the window velocity is dead from here; its LDS holds the spread force
+  const float Ux = (float)s_body[2], Uy = (float)s_body[3];
+  __syncthreads();   // the window velocity is dead from here; the grid accumulates the spread force
+  for (int k = tid; k < 2 * SS; k += nt) gflat[k] = 0.f;
 
-  // multi-direct forcing (ib/mdf.py:45-64); pass n_iter spreads the total force
-  for (int it = 0; it <= ib.n_iter; ++it) {
-    const bool last = it == ib.n_iter;
+  // multi-direct forcing (ib/mdf.py:45-64)
+  for (int it = 0; it < ib.n_iter; ++it) {
     float sfx[MPT], sfy[MPT];
 #pragma unroll
     for (int q = 0; q < MPT; ++q) {
-      if (!last) {
-        const float dfx = (Ux - ubx[q]) * ds2[q], dfy = (Uy - uby[q]) * ds2[q];
-        Fx[q] += dfx;
-        Fy[q] += dfy;
-        sfx[q] = dfx;
-        sfy[q] = dfy;
-      } else {
-        sfx[q] = Fx[q];
-        sfy[q] = Fy[q];
-      }
+      sfx[q] = (Ux - ubx[q]) * ds2[q];
+      sfy[q] = (Uy - uby[q]) * ds2[q];
+      Fx[q] += sfx[q];
+      Fy[q] += sfy[q];
     }
     if constexpr (BINS) {
-      if (mk[0] < M && regular[0]) {
-        float *r = rec + 12 * pos[mk[0]];
-        r[10] = sfx[0];
-        r[11] = sfy[0];
-      }
-      __syncthreads();
-      bin_spread(ug, rec, bptr, S);
-      __syncthreads();
-    } else {
-      for (int k = tid; k < 2 * SS; k += nt) ug[k] = 0.f;
-      __syncthreads();
+      if (mk[0] < M && regular[0]) rforce[my_pos] = make_float2(sfx[0], sfy[0]);
     }
-    if (s_irregular) {
+    __syncthreads();
+    if constexpr (BINS) {
+      if (SEG && use_seg) seg_spread(grid, rforce, sg, fcell, lcell, headp);
+      else bin_spread<SPREAD_RUNS>(grid, rec, M, bend, act, n_act, S);
+      if (s_irregular) __syncthreads();
+    }
+    if (!BINS || s_irregular) {
 #pragma unroll
       for (int q = 0; q < MPT; ++q) {
         if (mk[q] < M && !regular[q]) {
-#pragma unroll
-          for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-              const int si = (int)scatter_index(base_idx[q] + i * S + j, SS);
-              if (si >= 0) {
-                const float w = wx[q][i] * wy[q][j];
-                atomicAdd(&ug[si], sfx[q] * w);
-                atomicAdd(&ug[SS + si], sfy[q] * w);
-              }
-            }
+          float wxq[4], wyq[4];
+          weights(q, wxq, wyq);
+          scatter_marker(gflat, base_idx[q], S, wxq, wyq, sfx[q], sfy[q]);
         }
-      }
-      __syncthreads();
-    }
-    if (last) break;
-#pragma unroll
-    for (int q = 0; q < MPT; ++q) {
-      if (mk[q] < M) {
-        float sx = 0.f, sy = 0.f;
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            const int gi = (int)gather_index(base_idx[q] + i * S + j, SS);
-            const float w = wx[q][i] * wy[q][j];
-            sx += w * (ug[gi] * 0.5f);
-            sy += w * (ug[SS + gi] * 0.5f);
-          }
-        ubx[q] += sx;
-        uby[q] += sy;
       }
     }
     __syncthreads();
+    if (it + 1 == ib.n_iter) break;
+#pragma unroll
+    for (int q = 0; q < MPT; ++q) {
+      if (mk[q] < M) {
+        float wxq[4], wyq[4];
+        weights(q, wxq, wyq);
+        const float2 v = interp_marker(grid, base_idx[q], S, wxq, wyq);
+        ubx[q] = ub0x[q] + 0.5f * v.x;
+        uby[q] = ub0y[q] + 0.5f * v.y;
+      }
+    }
   }
+  if (ib.n_iter == 0) __syncthreads();
 
   // force window -> global
-  for (int k = tid; k < 2 * SS; k += nt) ib.g_win[k] = ug[k];
+  for (int k = tid; k < SS; k += nt) {
+    const float2 g = grid[k];
+    ib.g_win[k] = g.x;
+    ib.g_win[SS + k] = g.y;
+  }
 
   // body force h = sum(-F) (FP64)
   double hx = 0.0, hy = 0.0;
@@ -460,11 +831,10 @@ __device__ __forceinline__ void wait_ready(const VsxIB2D &ib, int target) {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
-// Window tile: publish the post-streaming velocity of its cells that lie in
-// the force window (into ib.g_win, read by the IB workgroup), then count the
-// tile in ib.win_origin[4 + (gstep & 1)] with an agent-scope release.
-__device__ __forceinline__ void publish_window_u(const VsxIB2D &ib, int gstep, bool valid, const int (&co)[3],
-                                                 const int (&org)[3], const float (&f)[9]) {
+// Window tile: store the post-streaming velocity of a cell that lies in the
+// force window into ib.g_win (read by the IB workgroup) ...
+__device__ __forceinline__ void store_window_u(const VsxIB2D &ib, bool valid, const int (&co)[3], const int (&org)[3],
+                                               const float (&f)[9]) {
   const int S = ib.win;
   const int r0 = co[0] - org[0], r1 = co[1] - org[1];
   if (valid && r0 >= 0 && r0 < S && r1 >= 0 && r1 < S) {
@@ -473,6 +843,11 @@ __device__ __forceinline__ void publish_window_u(const VsxIB2D &ib, int gstep, b
     ib.g_win[r0 * S + r1] = u[0];
     ib.g_win[S * S + r0 * S + r1] = u[1];
   }
+}
+
+// ... then count the tile in ib.win_origin[4 + (gstep & 1)] with an
+// agent-scope release.
+__device__ __forceinline__ void count_window_tile(const VsxIB2D &ib, int gstep) {
   __syncthreads();
   if (threadIdx.x == 0) {
     __threadfence();
