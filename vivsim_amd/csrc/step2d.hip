@@ -61,13 +61,17 @@ k_viv_fused(StepCtx c_arg, VsxIB2D ib, const typename SRC::T *__restrict__ src, 
       else pull<D2Q9, SRC>(c, src, co[0], co[1], 0, f);
     }
     store_window_u(ib, valid, co, org, f);
-    count_window_tile(ib, gstep);
+    count_window_tile(ib, gstep, (wx1 - wx0 + 1) * (wy1 - wy0 + 1));
     wait_ready(ib, gstep + 1);
     if (!valid) return;
     finish<D2Q9, COLL>(c, cell, co, org, true, f);
     store_cell<D2Q9, DST>(c, dst, cell, f);
     return;
   }
+  // bulk tile: wait (bounded) until the window tiles have published the window
+  // velocity, so that their pulls are not queued behind the bulk traffic
+  if (threadIdx.x == 0) spin_until(ib, &ib.win_origin[6], gstep + 1);
+  __syncthreads();
   if (!valid) return;
   const int cls = c.p.n_walls > 0 ? wall_class<D2Q9>(c, co) : -1;
   if (cls >= 0) {

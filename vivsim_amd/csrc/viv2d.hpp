@@ -54,11 +54,13 @@ __host__ __device__ inline int imax(int a, int b) { return a > b ? a : b; }
 //                              entry offsets of the active cells
 //   with BINS (one marker per thread), markers at their bin-order position p
 //   (markers whose stencil leaves the window at the end):
-//   rec   [max(10 M, SS + 1)]  16-byte aligned: wx [4][M], wy [4][M] stencil
+//   rec   [max(10.5 M, SS + 1)] 16-byte aligned: wx [4][M], wy [4][M] stencil
 //                              weights, force [M] float2 (the force being
-//                              spread).  While the bins are built it holds the
-//                              int32 bin counters.
-//   bend  uint16 [SS + 1]      end of each bin in bin order
+//                              spread), bin [M] uint16 (base cell of each
+//                              binned marker).  While the bins are built it
+//                              holds the int32 bin counters.
+//   bend  uint16 [3 S + 4 + SS + 1]  end of each bin in bin order, after 3 S + 4
+//                              zeros (bins with a negative index are empty)
 //   list  [max(M, SS / 2)]     int32 markers by bin while the bins are built, then
 //                              the uint16 list of active window cells
 //   seg   (over bend and list once the segmented spread is built) first and
@@ -67,14 +69,16 @@ __host__ __device__ inline int imax(int a, int b) { return a > b ? a : b; }
 struct IbLds {
   int rec, bend, list, seg, total;
 };
+// zeros in front of the bin ends: bend[i] = 0 for -bend_pad(S) <= i < 0
+__host__ __device__ inline int bend_pad(int S) { return 3 * S + 4; }
 __host__ __device__ inline IbLds ib_lds_layout(int S, int M) {
   const int SS = S * S;
   IbLds L;
   L.rec = (2 * SS + 3) & ~3;
-  L.bend = (L.rec + imax(10 * M, SS + 1) + 3) & ~3;
-  L.list = L.bend + (SS + 2) / 2;
+  L.bend = (L.rec + imax(10 * M + (M + 1) / 2, SS + 1) + 3) & ~3;
+  L.list = L.bend + (bend_pad(S) + SS + 2) / 2;
   L.seg = L.bend;
-  L.total = L.bend + imax((SS + 2) / 2 + imax(M, (SS + 1) / 2), 3 * kIbThreads);
+  L.total = L.bend + imax(L.list - L.bend + imax(M, (SS + 1) / 2), 3 * kIbThreads);
   return L;
 }
 __host__ __device__ inline int ib_lds_words(int S, int M, bool bins) {
@@ -120,13 +124,12 @@ __device__ __forceinline__ void block_exclusive_scan(int *a, int n, int *s_wsum)
 // negative index are empty).  Stencil point (a, 3 - j) of that marker is cell k.
 __device__ __forceinline__ void bin_run(const uint16_t *bend, int k, int a, int S, int &e0, int &n, int &b1, int &b2,
                                         int &b3) {
-  const int hi = k - a * S, lo = hi - 3;
-  const int e1 = hi >= 0 ? bend[hi] : 0;
-  e0 = hi >= 0 && lo > 0 ? bend[lo - 1] : 0;
-  n = e1 - e0;
-  b1 = lo >= 0 ? bend[lo] : 0;
-  b2 = lo >= -1 ? bend[lo + 1] : 0;
-  b3 = lo >= -2 ? bend[lo + 2] : 0;
+  const int lo = k - a * S - 3;
+  e0 = bend[lo - 1];
+  n = bend[lo + 3] - e0;
+  b1 = bend[lo];
+  b2 = bend[lo + 1];
+  b3 = bend[lo + 2];
 }
 
 // Window cells that receive force from a binned marker (a non-empty bin run),
@@ -142,8 +145,8 @@ __device__ __forceinline__ int active_cells(const uint16_t *bend, uint16_t *act,
       int tot = 0;
 #pragma unroll
       for (int a = 0; a < 4; ++a) {
-        const int hi = k - a * S, lo = hi - 3;
-        tot += hi >= 0 ? bend[hi] - (lo > 0 ? bend[lo - 1] : 0) : 0;
+        const int hi = k - a * S;
+        tot += bend[hi] - bend[hi - 4];
       }
       on = tot != 0;
     }
@@ -211,41 +214,50 @@ __device__ __forceinline__ void bin_spread(float2 *grid, const float *rec, int M
 // ------------------------------------------------------------------ segmented spread
 // The 16 stencil entries of every binned marker, ordered by window cell (cells
 // ascending; within a cell runs a = 0..3, markers in bin order, as bin_spread
-// sums them), kSpreadEPT consecutive entries per thread with their weights in
-// registers.  A spread is then kSpreadEPT force loads and FMAs per thread plus
-// a segmented sum: the thread holding the first entry of a cell owns it and
-// adds the partial sums of the following threads whose entries continue that
-// cell, in thread order.  Deterministic, no atomics, balanced over the threads.
+// sums them), kSpreadEPT consecutive entries per thread in registers, 16 bits
+// each: marker position e (10 bits) and stencil point (a, b) (2 + 2 bits),
+// with their weights.  A spread reads the force and the base cell of each
+// entry's marker (the cell is base + a S + b) and adds the products as a
+// segmented sum: the thread holding the first
+// entry of a cell owns it and adds the partial sums of the following threads
+// whose entries continue that cell, in thread order.  Deterministic, no
+// atomics, balanced over the threads.
 constexpr int kSpreadEPT = 8;
 
 struct SpreadSeg {
-  uint32_t ec[kSpreadEPT];   // marker position | cell << 16
-  float w[kSpreadEPT];       // stencil weight wx[a] * wy[b]
-  int nthr;                  // threads holding entries
+  uint32_t eab[kSpreadEPT / 2];   // entry t in bits 16 (t & 1) .. 16 (t & 1) + 15
+  float w[kSpreadEPT];            // stencil weight wx[a] * wy[b] (0 for padding)
+  int nthr;                       // threads holding entries
 };
 
-__device__ __forceinline__ int seg_cell(uint32_t ec) { return (int)(ec >> 16); }
+__device__ __forceinline__ int seg_entry(const SpreadSeg &sg, int t) {
+  return (int)((sg.eab[t >> 1] >> (16 * (t & 1))) & 0xffffu);
+}
 
 // Entry count of window cell k (its 4 bin runs).
 __device__ __forceinline__ int cell_entries(const uint16_t *bend, int k, int S) {
   int tot = 0;
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
-    const int hi = k - a * S, lo = hi - 3;
-    tot += hi >= 0 ? bend[hi] - (lo > 0 ? bend[lo - 1] : 0) : 0;
+    const int hi = k - a * S;
+    tot += bend[hi] - bend[hi - 4];
   }
   return tot;
 }
 
 // Builds the entries (every thread calls it; ends with a barrier).  Returns
 // false when the binned markers have more entries than the workgroup holds
-// (16 n > kSpreadEPT nt); the caller then uses bin_spread.  act / rptr: uint16
-// scratch for the active cells and their first entry; fcell, lcell: first and
+// (16 n > kSpreadEPT nt); the caller then uses bin_spread.  tmp: uint16
+// scratch of 2 SS + 2 (entry offset of each active cell, entry count of each
+// cell); act: uint16 scratch for the active cells; fcell, lcell: first and
 // last cell of each thread's entries (written after the entries are built, so
-// they may overlay bend and act).
+// they may overlay bend and act).  rec holds the weights and, after the
+// forces, the base cell of each binned marker (uint16).
 __device__ __forceinline__ bool seg_build(const float *rec, int M, const uint16_t *bend, int S, uint16_t *act,
-                                          uint16_t *rptr, uint16_t *fcell, uint16_t *lcell, int *s_wsum, SpreadSeg &sg) {
+                                          uint16_t *tmp, uint16_t *fcell, uint16_t *lcell, int *s_wsum, SpreadSeg &sg) {
   const int SS = S * S, tid = threadIdx.x, nt = blockDim.x;
+  const uint16_t *binof = reinterpret_cast<const uint16_t *>(rec + 10 * M);
+  uint16_t *rptr = tmp, *ccount = tmp + SS + 2;
   const int T = 16 * (int)bend[SS - 1];
   sg.nthr = (T + kSpreadEPT - 1) / kSpreadEPT;
   if (T > kSpreadEPT * nt) return false;   // uniform: every thread reads the same bend entry
@@ -254,7 +266,10 @@ __device__ __forceinline__ bool seg_build(const float *rec, int M, const uint16_
   const int chunk = (SS + nt - 1) / nt, k0 = min(tid * chunk, SS), k1 = min(k0 + chunk, SS);
   int mine = 0;
   for (int k = k0; k < k1; ++k) {
-    const int e = cell_entries(bend, k, S);
+    int e = 0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) e += bend[k - a * S] - bend[k - a * S - 4];
+    ccount[k] = (uint16_t)e;
     mine += e ? (e << 16) + 1 : 0;
   }
   int v = mine;
@@ -263,120 +278,132 @@ __device__ __forceinline__ bool seg_build(const float *rec, int M, const uint16_
     const int t = __shfl_up(v, o, 64);
     if ((tid & 63) >= o) v += t;
   }
-  if ((tid & 63) == 63) s_wsum[tid >> 6] = v;
   __shared__ int s_nact;
+  __shared__ uint16_t s_tcell[kIbThreads];   // active cell holding each thread's first entry
+  if ((tid & 63) == 63) s_wsum[tid >> 6] = v;
   __syncthreads();
-  if (tid == 0) {
-    int run = 0;
-    for (int w = 0; w < (nt + 63) / 64; ++w) {
-      const int x = s_wsum[w];
-      s_wsum[w] = run;
-      run += x;
+  if (tid < 64) {   // exclusive scan of the wave totals by the first wavefront
+    const int nw = (nt + 63) / 64;
+    const int x = tid < nw ? s_wsum[tid] : 0;
+    int y = x;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int t = __shfl_up(y, o, 64);
+      if (tid >= o) y += t;
     }
-    s_nact = run & 0xffff;
-    rptr[s_nact] = (uint16_t)T;   // end of the last active cell
+    if (tid < nw) s_wsum[tid] = y - x;
+    if (tid == nw - 1) {
+      s_nact = y & 0xffff;
+      rptr[y & 0xffff] = (uint16_t)T;   // end of the last active cell
+    }
   }
   __syncthreads();
   const int at = s_wsum[tid >> 6] + v - mine;
   int q = at & 0xffff, r = at >> 16;
   for (int k = k0; k < k1; ++k) {
-    const int e = cell_entries(bend, k, S);
+    const int e = ccount[k];
     if (e) {
       act[q] = (uint16_t)k;
       rptr[q] = (uint16_t)r;
+      for (int i = (r + kSpreadEPT - 1) / kSpreadEPT; i * kSpreadEPT < r + e; ++i) s_tcell[i] = (uint16_t)q;
       ++q;
       r += e;
     }
   }
   __syncthreads();
-  // this thread's entries [g0, g0 + kSpreadEPT): the active cell holding g0 is
-  // the largest cq with rptr[cq] <= g0 (rptr ascending, rptr[n_act] = T)
+  // this thread's entries [g0, g0 + kSpreadEPT), walked with the run lengths
+  // and first markers of the current cell in registers; the cell advances when
+  // an entry reaches the next cell's first entry
   const int g0 = tid * kSpreadEPT;
-  int cq = 0;
-  if (g0 < T) {
-    int l = 0, h = s_nact;
-    while (h - l > 1) {
-      const int m = (l + h) >> 1;
-      if ((int)rptr[m] <= g0) l = m;
-      else h = m;
-    }
-    cq = l;
-  }
-  // walk the entries: the cell of entry g is advanced when g reaches the next
-  // cell's first entry; the run of an entry and its marker are re-read from
-  // bend (once per step)
-  int k = 0, start = 0, next = 0x7fffffff;
-  if (g0 < T) {
+  int cq = 0, k = 0, start = 0, next = 0x7fffffff, n[4] = {0, 0, 0, 0}, e0[4] = {0, 0, 0, 0};
+  auto load_cell = [&]() {
     k = act[cq];
     start = rptr[cq];
     next = rptr[cq + 1];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      e0[a] = bend[k - a * S - 4];
+      n[a] = bend[k - a * S] - e0[a];
+    }
+  };
+  if (g0 < T) {
+    cq = s_tcell[tid];
+    load_cell();
   }
+  int kfirst = 0;
+#pragma unroll
+  for (int t = 0; t < kSpreadEPT / 2; ++t) sg.eab[t] = 0;
 #pragma unroll
   for (int t = 0; t < kSpreadEPT; ++t) {
+    sg.w[t] = 0.f;
+    if (g0 >= T) continue;   // no entries (the thread takes no part in the spread)
     const int g = min(g0 + t, T - 1);   // past the end: the last entry again, with weight 0
     if (g == next) {
       ++cq;
-      k = act[cq];
-      start = next;
-      next = rptr[cq + 1];
+      load_cell();
     }
-    int o = g - start, a = 0, e0 = 0, b1 = 0, b2 = 0, b3 = 0;
+    int o = g - start, a = 0;
 #pragma unroll
-    for (int aa = 0; aa < 4; ++aa) {
-      int ea, na, c1, c2, c3;
-      bin_run(bend, k, aa, S, ea, na, c1, c2, c3);
-      const bool here = o >= 0 && o < na;
-      a = here ? aa : a;
-      e0 = here ? ea : e0;
-      b1 = here ? c1 : b1;
-      b2 = here ? c2 : b2;
-      b3 = here ? c3 : b3;
-      e0 = here ? e0 + o : e0;
-      o -= na;
+    for (int aa = 0; aa < 3; ++aa) {
+      const bool past = a == aa && o >= n[aa];
+      o -= past ? n[aa] : 0;
+      a += past ? 1 : 0;
     }
-    const int e = e0;
-    const int j = (e >= b1) + (e >= b2) + (e >= b3);
-    sg.ec[t] = (uint32_t)e | ((uint32_t)k << 16);
-    sg.w[t] = g0 + t < T ? rec[a * M + e] * rec[(7 - j) * M + e] : 0.f;   // wx[a] * wy[3 - j]
+    const int e = (a == 0 ? e0[0] : a == 1 ? e0[1] : a == 2 ? e0[2] : e0[3]) + o;
+    // stencil point (a, b) of marker e is cell k: b = k - a S - (its base cell)
+    const int b = k - a * S - (int)binof[e];
+    const uint32_t v = (uint32_t)e | ((uint32_t)a << 10) | ((uint32_t)b << 12);
+    sg.eab[t >> 1] |= v << (16 * (t & 1));
+    const float wxa = rec[a * M + e];
+    const float wy0 = rec[4 * M + e], wy1 = rec[5 * M + e], wy2 = rec[6 * M + e], wy3 = rec[7 * M + e];
+    sg.w[t] = g0 + t < T ? wxa * (b == 0 ? wy0 : b == 1 ? wy1 : b == 2 ? wy2 : wy3) : 0.f;
+    if (t == 0) kfirst = k;
   }
   __syncthreads();   // bend, act and rptr are dead from here
   if (tid < sg.nthr) {
-    fcell[tid] = (uint16_t)seg_cell(sg.ec[0]);
-    lcell[tid] = (uint16_t)seg_cell(sg.ec[kSpreadEPT - 1]);
+    fcell[tid] = (uint16_t)kfirst;
+    lcell[tid] = (uint16_t)k;
   }
   __syncthreads();
   return true;
 }
 
 // One spread of the forces (rec force [M] float2) added onto the grid.
-__device__ __forceinline__ void seg_spread(float2 *grid, const float2 *force, const SpreadSeg &sg,
+__device__ __forceinline__ void seg_spread(float2 *grid, const float *rec, int M, int S, const SpreadSeg &sg,
                                            const uint16_t *fcell, const uint16_t *lcell, float2 *headp) {
   const int tid = threadIdx.x;
+  const float2 *force = reinterpret_cast<const float2 *>(rec + 8 * M);
+  const uint16_t *binof = reinterpret_cast<const uint16_t *>(rec + 10 * M);
   int tail_cell = -1;
   float2 tail = make_float2(0.f, 0.f);
   if (tid < sg.nthr) {
     const bool cont_prev = tid > 0 && fcell[tid] == lcell[tid - 1];
     const bool cont_next = tid + 1 < sg.nthr && lcell[tid] == fcell[tid + 1];
-    // the entries are made opaque here: addresses derived from them and kept
-    // across the MDF loop would take registers of their own.  Forces are
-    // loaded half the entries at a time (register budget).
     float sx = 0.f, sy = 0.f;
     bool first = true;
-    int cur = seg_cell(sg.ec[0]);
+    int cur = fcell[tid];
+    // the entries are made opaque here: what is derived from them (addresses,
+    // cells) would otherwise be kept across the MDF loop in registers
+    SpreadSeg es = sg;
+#pragma unroll
+    for (int t = 0; t < kSpreadEPT / 2; ++t) asm volatile("" : "+v"(es.eab[t]));
+    // half the entries' loads in flight at a time (register budget)
 #pragma unroll
     for (int h = 0; h < kSpreadEPT; h += kSpreadEPT / 2) {
-      uint32_t ec[kSpreadEPT / 2];
+      int cell[kSpreadEPT / 2];
+      float w[kSpreadEPT / 2];
       float2 fm[kSpreadEPT / 2];
 #pragma unroll
       for (int t = 0; t < kSpreadEPT / 2; ++t) {
-        ec[t] = sg.ec[h + t];
-        asm volatile("" : "+v"(ec[t]));
-        fm[t] = force[ec[t] & 0xffffu];
+        const int v = seg_entry(es, h + t);
+        const int e = v & 1023, a = (v >> 10) & 3, b = (v >> 12) & 3;
+        fm[t] = force[e];
+        w[t] = sg.w[h + t];
+        cell[t] = (int)binof[e] + a * S + b;
       }
 #pragma unroll
       for (int t = 0; t < kSpreadEPT / 2; ++t) {
-        const int ct = seg_cell(ec[t]);
-        if (h + t > 0 && ct != cur) {   // the segment of cur ends inside this thread
+        if (cell[t] != cur) {   // the segment of cur ends inside this thread
           if (first && cont_prev) {
             headp[tid] = make_float2(sx, sy);
           } else {
@@ -387,10 +414,10 @@ __device__ __forceinline__ void seg_spread(float2 *grid, const float2 *force, co
           }
           first = false;
           sx = sy = 0.f;
-          cur = ct;
+          cur = cell[t];
         }
-        sx += sg.w[h + t] * fm[t].x;
-        sy += sg.w[h + t] * fm[t].y;
+        sx += w[t] * fm[t].x;
+        sy += w[t] * fm[t].y;
       }
       asm volatile("" : "+v"(sx), "+v"(sy));
     }
@@ -421,11 +448,13 @@ __device__ __forceinline__ void seg_spread(float2 *grid, const float2 *force, co
   }
 }
 
-// Bounded wait of thread 0 until *flag >= target (agent-scope acquire); on
-// time-out the error flag ib.win_origin[3] is set and the wait ends.
+// Bounded wait of thread 0 until *flag >= target (relaxed agent-scope loads:
+// an acquire per poll would invalidate the L2 every time; callers that read
+// data published with the flag follow the wait with an agent-scope acquire
+// fence).  On time-out the error flag ib.win_origin[3] is set and the wait ends.
 __device__ __forceinline__ void spin_until(const VsxIB2D &ib, int32_t *flag, int target) {
   int spins = 0;
-  while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+  while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
     __builtin_amdgcn_s_sleep(2);
     if (++spins > kSpinLimit) {
       __hip_atomic_store(&ib.win_origin[3], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -539,7 +568,7 @@ __device__ __forceinline__ void ib_block(const StepCtx &c, const VsxIB2D &ib, co
   float *rec = lds + lay.rec;
   float2 *rforce = reinterpret_cast<float2 *>(rec + 8 * M);
   int *cnt = reinterpret_cast<int *>(rec);
-  uint16_t *bend = reinterpret_cast<uint16_t *>(lds + lay.bend);
+  uint16_t *bend = reinterpret_cast<uint16_t *>(lds + lay.bend) + bend_pad(S);
   int *blist = reinterpret_cast<int *>(lds + lay.list);
   uint16_t *act = reinterpret_cast<uint16_t *>(lds + lay.list);
   __shared__ double s_body[6];
@@ -638,7 +667,7 @@ __device__ __forceinline__ void ib_block(const StepCtx &c, const VsxIB2D &ib, co
     } else if (live) {
       my_pos = M - 1 - atomicAdd(&s_nirr, 1);
     }
-    for (int k = tid; k < SS; k += nt) bend[k] = (uint16_t)cnt[k];
+    for (int k = tid - bend_pad(S); k < SS; k += nt) bend[k] = k < 0 ? 0 : (uint16_t)cnt[k];
     __syncthreads();   // counters and marker list are dead from here
     if (live) {
 #pragma unroll
@@ -646,6 +675,7 @@ __device__ __forceinline__ void ib_block(const StepCtx &c, const VsxIB2D &ib, co
         rec[i * M + my_pos] = wx[0][i];
         rec[(4 + i) * M + my_pos] = wy[0][i];
       }
+      if (binned) reinterpret_cast<uint16_t *>(rec + 10 * M)[my_pos] = (uint16_t)base_idx[0];
     }
     if constexpr (SEG)
       use_seg = seg_build(rec, M, bend, S, act, reinterpret_cast<uint16_t *>(lds), fcell, lcell, s_wsum, sg);
@@ -731,7 +761,7 @@ __device__ __forceinline__ void ib_block(const StepCtx &c, const VsxIB2D &ib, co
     }
     __syncthreads();
     if constexpr (BINS) {
-      if (SEG && use_seg) seg_spread(grid, rforce, sg, fcell, lcell, headp);
+      if (SEG && use_seg) seg_spread(grid, rec, M, S, sg, fcell, lcell, headp);
       else bin_spread<SPREAD_RUNS>(grid, rec, M, bend, act, n_act, S);
       if (s_irregular) __syncthreads();
     }
@@ -846,12 +876,17 @@ __device__ __forceinline__ void store_window_u(const VsxIB2D &ib, bool valid, co
 }
 
 // ... then count the tile in ib.win_origin[4 + (gstep & 1)] with an
-// agent-scope release.
-__device__ __forceinline__ void count_window_tile(const VsxIB2D &ib, int gstep) {
+// agent-scope release; the last of the n_wtiles window tiles also raises
+// ib.win_origin[6] to gstep + 1 (the bulk tiles start their own traffic after
+// it, so the window pulls are served first).
+__device__ __forceinline__ void count_window_tile(const VsxIB2D &ib, int gstep, int n_wtiles) {
   __syncthreads();
   if (threadIdx.x == 0) {
     __threadfence();
-    __hip_atomic_fetch_add(&ib.win_origin[4 + (gstep & 1)], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    const int before =
+        __hip_atomic_fetch_add(&ib.win_origin[4 + (gstep & 1)], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (before == n_wtiles - 1)
+      __hip_atomic_store(&ib.win_origin[6], gstep + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
