@@ -537,6 +537,23 @@ __device__ __forceinline__ void window_store(float2 *grid, int base, const float
   }
 }
 
+// Wait (thread 0, bounded) until the n_wtiles window tiles of step gstep have
+// published the window velocity in ib.g_win, reset their counter for step
+// gstep + 2, then acquire at agent scope.
+__device__ __forceinline__ void wait_window_u(const VsxIB2D &ib, int gstep, int n_wtiles) {
+  if (threadIdx.x == 0) {
+    int32_t *cnt_u = &ib.win_origin[4 + (gstep & 1)];
+    spin_until(ib, cnt_u, n_wtiles);
+    __hip_atomic_store(cnt_u, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+// window values per thread held in registers between the wait for the window
+// velocity and its store into LDS (windows with 2 S^2 <= kUPrefetch threads)
+constexpr int kUPrefetch = 8;
+
 // One workgroup per body (examples/2d/vortex_induced_vibration.py:99-137):
 //   window u     macroscopic velocity of the post-streaming state (pull + walls);
 //                U_GLOBAL: published in ib.g_win by the n_wtiles tiles that
@@ -643,7 +660,8 @@ __device__ __forceinline__ void ib_block(const StepCtx &c, const VsxIB2D &ib, co
   }
 
   int my_pos = 0, n_act = 0;
-  bool use_seg = false;
+  bool use_seg = false, u_pref = false;
+  float upf[kUPrefetch];
   SpreadSeg sg;
   uint16_t *fcell = reinterpret_cast<uint16_t *>(lds + lay.seg), *lcell = fcell + kIbThreads;
   float2 *headp = reinterpret_cast<float2 *>(lds + lay.seg + kIbThreads);
@@ -677,6 +695,19 @@ __device__ __forceinline__ void ib_block(const StepCtx &c, const VsxIB2D &ib, co
       }
       if (binned) reinterpret_cast<uint16_t *>(rec + 10 * M)[my_pos] = (uint16_t)base_idx[0];
     }
+    if constexpr (U_GLOBAL) {
+      // the window velocity is published early (the bulk tiles wait for it):
+      // its loads are issued now and stay in registers over the entry build
+      wait_window_u(ib, gstep, n_wtiles);
+      u_pref = 2 * SS <= kUPrefetch * nt;
+      if (u_pref) {
+#pragma unroll
+        for (int j = 0; j < kUPrefetch; ++j) {
+          const int w = tid + j * nt;   // interleaved grid word: cell w / 2, component w % 2
+          upf[j] = w < 2 * SS ? ib.g_win[(w & 1) * SS + (w >> 1)] : 0.f;
+        }
+      }
+    }
     if constexpr (SEG)
       use_seg = seg_build(rec, M, bend, S, act, reinterpret_cast<uint16_t *>(lds), fcell, lcell, s_wsum, sg);
     if (!use_seg) n_act = active_cells(bend, act, S, &s_nact);   // ends with a barrier
@@ -684,14 +715,16 @@ __device__ __forceinline__ void ib_block(const StepCtx &c, const VsxIB2D &ib, co
 
   // window velocity
   if constexpr (U_GLOBAL) {
-    if (tid == 0) {
-      int32_t *cnt_u = &ib.win_origin[4 + (gstep & 1)];
-      spin_until(ib, cnt_u, n_wtiles);
-      __hip_atomic_store(cnt_u, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);   // reused two steps later
+    if (u_pref) {
+#pragma unroll
+      for (int j = 0; j < kUPrefetch; ++j) {
+        const int w = tid + j * nt;
+        if (w < 2 * SS) gflat[w] = upf[j];
+      }
+    } else {
+      if (!BINS) wait_window_u(ib, gstep, n_wtiles);
+      for (int k = tid; k < SS; k += nt) grid[k] = make_float2(ib.g_win[k], ib.g_win[SS + k]);
     }
-    __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    for (int k = tid; k < SS; k += nt) grid[k] = make_float2(ib.g_win[k], ib.g_win[SS + k]);
   } else {
     for (int base = 0; base < SS; base += UW * nt) {
       float fu[UW][9];
