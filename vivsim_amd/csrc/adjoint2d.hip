@@ -216,8 +216,31 @@ struct AdjIB {
   double inv_T;
 };
 
-// KEEP: the forward replay's spread grids A_k stay in LDS for the reverse loop
-// (needs (4 + 2 NIT) S^2 floats of LDS), else they are spread again.
+// LDS of k_adj_ib in 4-byte words (S = window side, SS = S * S, M = markers;
+// grids are float2 [SS], x and y of a cell side by side):
+//   uw [SS] window velocity of f_t, G [SS] spread grid, KEEP: A [NIT][SS] the
+//   forward replay's spreads; then the binned-marker tables of the spread
+//   (viv2d.hpp): rec [max(10 M, SS + 1)], bend (zero-padded bin ends), list.
+struct AdjLds {
+  int G, A, rec, bend, list, total;
+};
+__host__ __device__ inline AdjLds adj_lds_layout(int S, int M, int nit, bool keep) {
+  const int SS = S * S;
+  AdjLds L;
+  L.G = 2 * SS;
+  L.A = 4 * SS;
+  L.rec = ((keep ? 4 + 2 * nit : 4) * SS + 3) & ~3;
+  L.bend = (L.rec + imax(10 * M, SS + 1) + 3) & ~3;
+  L.list = L.bend + (bend_pad(S) + SS + 2) / 2;
+  L.total = L.list + imax(M, (SS + 1) / 2);
+  return L;
+}
+
+// KEEP: the forward replay's spread grids A_k stay in LDS for the reverse loop,
+// else they are spread again.  Spreads: markers binned by stencil base cell
+// once per step, each spread a deterministic gather over the active window
+// cells (bin_spread); markers whose stencil leaves the window use LDS atomics
+// with the index rule of the operation (scatter or gather rule).
 template <int NIT, bool KEEP>
 __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const float *__restrict__ ft,
                                                  const float *__restrict__ gbar, float *__restrict__ fbar,
@@ -228,16 +251,23 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
   const int S = ib.win, SS = S * S, M = ib.n_markers;
   const int tid = threadIdx.x, nt = blockDim.x;
   const size_t n = (size_t)P.nx * P.ny;
-  float *uw = lds;            // [2][SS] window velocity of f_t
-  float *G = lds + 2 * SS;    // [2][SS] spread grid
-  float *Ak = lds + 4 * SS;   // KEEP: [NIT][2][SS] forward spreads
+  const AdjLds lay = adj_lds_layout(S, M, NIT, KEEP);
+  float2 *uw = reinterpret_cast<float2 *>(lds);            // window velocity of f_t
+  float2 *G = reinterpret_cast<float2 *>(lds + lay.G);     // spread grid
+  float2 *Ak = reinterpret_cast<float2 *>(lds + lay.A);    // KEEP: [NIT] forward spreads
+  float *rec = lds + lay.rec;
+  int *cnt = reinterpret_cast<int *>(rec);
+  uint16_t *bend = reinterpret_cast<uint16_t *>(lds + lay.bend) + bend_pad(S);
+  int *blist = reinterpret_cast<int *>(lds + lay.list);
+  uint16_t *actl = reinterpret_cast<uint16_t *>(lds + lay.list);
   __shared__ double s_body[6], s_ab[6], s_misc[4];
   __shared__ double red[16];
-  __shared__ int s_org[4];
+  __shared__ int s_org[4], s_wsum[kIbThreads / 64], s_irr, s_nirr, s_nact;
   if (tid < 6) {
     s_body[tid] = body[tid];
     s_ab[tid] = adj_body[tid];
   }
+  if (tid == 0) s_irr = s_nirr = 0;
   __syncthreads();
   if (tid == 0) {
     // objective term and Newmark adjoint (dyn.py:5-51), per DOF
@@ -282,14 +312,14 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
       jx += (float)D2Q9::cx[i] * v;
       jy += (float)D2Q9::cy[i] * v;
     }
-    uw[k] = jx / rho;
-    uw[SS + k] = jy / rho;
+    uw[k] = make_float2(jx / rho, jy / rho);
   }
   // marker of this thread (lane-major, as the forward) and its stencil + derivatives
   const int m = (tid & 63) * (nt >> 6) + (tid >> 6);
   const bool act = m < M;
   float wx[4] = {0, 0, 0, 0}, wy[4] = {0, 0, 0, 0}, dwx[4] = {0, 0, 0, 0}, dwy[4] = {0, 0, 0, 0};
   int base = 0;
+  bool regular = false;
   float ds2 = 0.f;
   if (act) {
     const float rx = (float)(ib.marker_x[m] + s_body[4] - (double)x0);
@@ -305,12 +335,44 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
     }
     base = (refx - 1) * S + (refy - 1);
     ds2 = (ib.ds_arr ? ib.ds_arr[m] : ib.ds) * 2.0f;
+    regular = refx - 1 >= 0 && refx - 1 <= S - 4 && refy - 1 >= 0 && refy - 1 <= S - 4;
+    if (!regular) s_irr = 1;
   }
+  // markers binned by base cell (count, scan, place, rank by marker index)
+  const bool binned = act && regular;
+  for (int k = tid; k <= SS; k += nt) cnt[k] = 0;
   __syncthreads();
+  if (binned) atomicAdd(&cnt[base], 1);
+  __syncthreads();
+  block_exclusive_scan(cnt, SS, s_wsum);
+  if (binned) blist[atomicAdd(&cnt[base], 1)] = m;
+  __syncthreads();
+  int my_pos = 0;
+  if (binned) {
+    const int s0 = base > 0 ? cnt[base - 1] : 0, e0 = cnt[base];
+    int rank = 0;
+    for (int i = s0; i < e0; ++i) rank += blist[i] < m;
+    my_pos = s0 + rank;
+  } else if (act) {
+    my_pos = M - 1 - atomicAdd(&s_nirr, 1);
+  }
+  for (int k = tid - bend_pad(S); k < SS; k += nt) bend[k] = k < 0 ? 0 : (uint16_t)cnt[k];
+  __syncthreads();   // counters and marker list are dead from here
+  if (binned) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      rec[i * M + my_pos] = wx[i];
+      rec[(4 + i) * M + my_pos] = wy[i];
+    }
+  }
+  const int n_act = active_cells(bend, actl, S, &s_nact);   // ends with a barrier
+  float2 *rforce = reinterpret_cast<float2 *>(rec + 8 * M);
+
   const float Ux = (float)s_body[2], Uy = (float)s_body[3];
   float rxb = 0.f, ryb = 0.f;   // adjoint of the marker position relative to the window
-  // gather of a 2-component grid at the stencil: value sums and W-bar terms with coefficients (cx, cy)
-  auto gather = [&](const float *grid, bool scatter_rule, float cx, float cy, float &vx, float &vy) {
+  // gather of a 2-component grid at the stencil: value sums and W-bar terms with
+  // coefficients (cx, cy); fetch(gi) returns the grid value of window cell gi
+  auto gather_with = [&](auto fetch, bool scatter_rule, float cx, float cy, float &vx, float &vy) {
     vx = vy = 0.f;
     if (!act) return;
 #pragma unroll
@@ -320,37 +382,46 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
         const int p = base + i * S + j;
         int gi;
         if (scatter_rule) {
-          gi = (int)scatter_index(p, SS);
+          gi = scatter_index32(p, SS);
           if (gi < 0) continue;
         } else {
-          gi = (int)gather_index(p, SS);
+          gi = gather_index32(p, SS);
         }
         const float w = wx[i] * wy[j];
-        const float gxv = grid[gi], gyv = grid[SS + gi];
-        vx += w * gxv;
-        vy += w * gyv;
-        const float wb = cx * gxv + cy * gyv;   // W-bar of this stencil point
+        const float2 gv = fetch(gi);
+        vx += w * gv.x;
+        vy += w * gv.y;
+        const float wb = cx * gv.x + cy * gv.y;   // W-bar of this stencil point
         rxb -= wb * dwx[i] * wy[j];
         ryb -= wb * wx[i] * dwy[j];
       }
   };
-  // spread of (vx, vy) into dst with the given index rule
-  auto spread_to = [&](float *dst, bool scatter_rule, float vx, float vy) {
-    for (int k = tid; k < 2 * SS; k += nt) dst[k] = 0.f;
+  auto gather = [&](const float2 *grid, bool scatter_rule, float cx, float cy, float &vx, float &vy) {
+    gather_with([&](int gi) { return grid[gi]; }, scatter_rule, cx, cy, vx, vy);
+  };
+  // dst = spread of (vx, vy) with the given index rule
+  auto spread_to = [&](float2 *dst, bool scatter_rule, float vx, float vy) {
+    for (int k = tid; k < SS; k += nt) dst[k] = make_float2(0.f, 0.f);
+    if (binned) rforce[my_pos] = make_float2(vx, vy);
     __syncthreads();
-    if (act) {
+    bin_spread<1>(dst, rec, M, bend, actl, n_act, S);
+    if (s_irr) {
+      __syncthreads();
+      if (act && !regular) {
+        float *dflat = reinterpret_cast<float *>(dst);
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int p = base + i * S + j;
-          const int gi = scatter_rule ? (int)scatter_index(p, SS) : (int)gather_index(p, SS);
-          if (gi >= 0) {
-            const float w = wx[i] * wy[j];
-            atomicAdd(&dst[gi], vx * w);
-            atomicAdd(&dst[SS + gi], vy * w);
+          for (int j = 0; j < 4; ++j) {
+            const int p = base + i * S + j;
+            const int gi = scatter_rule ? scatter_index32(p, SS) : gather_index32(p, SS);
+            if (gi >= 0) {
+              const float w = wx[i] * wy[j];
+              atomicAdd(&dflat[2 * gi], vx * w);
+              atomicAdd(&dflat[2 * gi + 1], vy * w);
+            }
           }
-        }
+      }
     }
     __syncthreads();
   };
@@ -364,19 +435,18 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
     dFy[k] = (Uy - uby) * ds2;
     Fx += dFx[k];
     Fy += dFy[k];
-    float *A = KEEP ? Ak + 2 * SS * k : G;
+    float2 *A = KEEP ? Ak + SS * k : G;
     spread_to(A, true, dFx[k], dFy[k]);
-    float ax, ay;
     if (act) {
-      ax = ay = 0.f;
+      float ax = 0.f, ay = 0.f;
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          const int gi = (int)gather_index(base + i * S + j, SS);
+          const float2 av = A[gather_index32(base + i * S + j, SS)];
           const float w = wx[i] * wy[j];
-          ax += w * (A[gi] * 0.5f);
-          ay += w * (A[SS + gi] * 0.5f);
+          ax += w * (av.x * 0.5f);
+          ay += w * (av.y * 0.5f);
         }
       ubx += ax;
       uby += ay;
@@ -386,15 +456,15 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
   // ---- reverse
   // g = W^T F (scatter rule); H = -sum F + a area
   float Fbx, Fby;
-  gather(gbar, true, Fx, Fy, Fbx, Fby);
+  gather_with([&](int gi) { return make_float2(gbar[gi], gbar[SS + gi]); }, true, Fx, Fy, Fbx, Fby);
   Fbx -= (float)s_misc[0];
   Fby -= (float)s_misc[1];
   float ubbx = 0.f, ubby = 0.f, Vbx = 0.f, Vby = 0.f;
 #pragma unroll
   for (int k = NIT - 1; k >= 0; --k) {
     // ub_{k+1} = ub_k + W (0.5 A_k), A_k = W^T dF_k
-    const float *A = G;
-    if constexpr (KEEP) A = Ak + 2 * SS * k;
+    const float2 *A = G;
+    if constexpr (KEEP) A = Ak + SS * k;
     else spread(true, dFx[k], dFy[k]);
     {
       float vx, vy;
@@ -419,16 +489,16 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
   spread(false, ubbx, ubby);
   // fbar_t += (c_i - u) . ubar / rho on the window cells
   for (int k = tid; k < SS; k += nt) {
-    const float ubx0 = G[k], uby0 = G[SS + k];
-    if (ubx0 == 0.f && uby0 == 0.f) continue;
+    const float2 ub0 = G[k];
+    if (ub0.x == 0.f && ub0.y == 0.f) continue;
     const size_t cell = (size_t)(xc + k / S) * P.ny + (yc + k % S);
     float rho = 0.f;
 #pragma unroll
     for (int i = 0; i < 9; ++i) rho += ft[i * n + cell];
-    const float ux = uw[k], uy = uw[SS + k];
+    const float2 u = uw[k];
 #pragma unroll
     for (int i = 0; i < 9; ++i)
-      fbar[i * n + cell] += (ubx0 * ((float)D2Q9::cx[i] - ux) + uby0 * ((float)D2Q9::cy[i] - uy)) / rho;
+      fbar[i * n + cell] += (ub0.x * ((float)D2Q9::cx[i] - u.x) + ub0.y * ((float)D2Q9::cy[i] - u.y)) / rho;
   }
   // marker-position adjoints: delta (radial direction), displacement
   if (act) grad_delta[m] += (double)rxb * dir_x[m] + (double)ryb * dir_y[m];
@@ -489,14 +559,14 @@ int validate_adj(const VsxLbm *p, const VsxIB2D *ib, const VsxAdj2D *a) {
 template <int NIT>
 void launch_adj_ib(const VsxIB2D &ib, const AdjIB &P, const float *ft, const float *gbar, float *fbar,
                    const double *body, const double *body1, const VsxAdj2D &a, int max_lds, hipStream_t s) {
-  const size_t SS = (size_t)ib.win * ib.win;
-  const size_t keep = sizeof(float) * (4 + 2 * NIT) * SS;
+  const size_t keep = sizeof(float) * (size_t)adj_lds_layout(ib.win, ib.n_markers, NIT, true).total;
+  const size_t drop = sizeof(float) * (size_t)adj_lds_layout(ib.win, ib.n_markers, NIT, false).total;
   if (keep + 2048 <= (size_t)max_lds)
     hipLaunchKernelGGL((k_adj_ib<NIT, true>), dim3(1), dim3(kIbThreads), keep, s, ib, P, ft, gbar, fbar, body, body1,
                        a.adj_body, a.grad_delta, a.dir_x, a.dir_y);
   else
-    hipLaunchKernelGGL((k_adj_ib<NIT, false>), dim3(1), dim3(kIbThreads), sizeof(float) * 4 * SS, s, ib, P, ft, gbar,
-                       fbar, body, body1, a.adj_body, a.grad_delta, a.dir_x, a.dir_y);
+    hipLaunchKernelGGL((k_adj_ib<NIT, false>), dim3(1), dim3(kIbThreads), drop, s, ib, P, ft, gbar, fbar, body, body1,
+                       a.adj_body, a.grad_delta, a.dir_x, a.dir_y);
 }
 
 int viv_adjoint(const VsxLbm *p, const VsxIB2D *ib, const VsxAdj2D *a, const float *f0, void *wa, void *wb,
@@ -511,7 +581,7 @@ int viv_adjoint(const VsxLbm *p, const VsxIB2D *ib, const VsxAdj2D *a, const flo
   int dev = 0, max_lds = 0;
   VSX_HIP(hipGetDevice(&dev));
   VSX_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-  VSX_REQUIRE(sizeof(float) * 4 * (size_t)ib->win * ib->win + 1024 <= (size_t)max_lds,
+  VSX_REQUIRE(sizeof(float) * (size_t)adj_lds_layout(ib->win, ib->n_markers, 1, false).total + 2048 <= (size_t)max_lds,
               "IB window %d too large for the adjoint workgroup", ib->win);
   hipStream_t s = as_stream(stream);
   const int T = a->n_steps, K = a->seg, nseg = (T + K - 1) / K;
