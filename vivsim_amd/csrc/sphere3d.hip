@@ -2,7 +2,7 @@
 //
 // Per step (examples/3d/flow_past_sphere.py:139-171, ib/mdf.py:10-64):
 //   k_win_u3d      window velocity of the post-streaming state          (cells)
-//   k_mark3d(0)    ub = interp(u); dF = (U - ub) 2 ds; F = dF             (markers)
+//   k_mark3d(0)    ub = interp(u); dF = (U - ub) 2 ds; F = dF    (a wavefront per marker)
 //   k_spread3d     A = spread(dF)                                        (cells)
 //   k_mark3d(k)    ub += interp(0.5 A); dF = ...; F += dF                (markers)
 //   ...            (n_iter - 1 times)
@@ -87,26 +87,31 @@ __global__ __launch_bounds__(256) void k_win_u3d(StepCtx c_arg, IB3DDev d, const
 // Marker phase `it` of the multi-direct forcing loop (ib/mdf.py:45-57).
 // it = 0: ub = interp(u_win), F = 0; it > 0: ub += interp(0.5 * acc).
 // Then, when it < n_iter: dF = (U - ub) * 2 ds, F += dF.
+// One wavefront per marker, one lane per stencil point (kS3 = 64): the
+// stencil and window loads of a marker are issued at once and summed by a
+// fixed butterfly over the lanes (deterministic).
+static_assert(kS3 == 64, "one lane per stencil point of a wavefront");
 __global__ __launch_bounds__(256) void k_mark3d(IB3DDev d, int it) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= d.M) return;
+  const int lane = threadIdx.x & 63;
+  const int m = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (m >= d.M) return;   // whole wavefronts
   const float *grid = it == 0 ? d.u_win : d.acc;
-  const float *W = d.W + (int64_t)m * kS3;
-  const int32_t *I = d.I + (int64_t)m * kS3;
-  float s0 = 0.f, s1 = 0.f, s2 = 0.f;
-  for (int j = 0; j < kS3; ++j) {
-    const int64_t gi = gather_index(I[j], d.cells);
-    const float w = W[j];
-    if (it == 0) {
-      s0 += w * grid[gi];
-      s1 += w * grid[d.cells + gi];
-      s2 += w * grid[2 * d.cells + gi];
-    } else {
-      s0 += w * (grid[gi] * 0.5f);
-      s1 += w * (grid[d.cells + gi] * 0.5f);
-      s2 += w * (grid[2 * d.cells + gi] * 0.5f);
-    }
+  const int64_t gi = gather_index(d.I[(int64_t)m * kS3 + lane], d.cells);
+  const float w = d.W[(int64_t)m * kS3 + lane];
+  float v0 = grid[gi], v1 = grid[d.cells + gi], v2 = grid[2 * d.cells + gi];
+  if (it != 0) {
+    v0 *= 0.5f;
+    v1 *= 0.5f;
+    v2 *= 0.5f;
   }
+  float s0 = w * v0, s1 = w * v1, s2 = w * v2;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    s0 += __shfl_xor(s0, o, 64);
+    s1 += __shfl_xor(s1, o, 64);
+    s2 += __shfl_xor(s2, o, 64);
+  }
+  if (lane != 0) return;
   float ub[3], F[3];
   if (it == 0) {
     ub[0] = s0, ub[1] = s1, ub[2] = s2;
@@ -144,7 +149,21 @@ __global__ __launch_bounds__(256) void k_spread3d(IB3DDev d, const float *__rest
   if (k < d.cells) {
     float s0 = 0.f, s1 = 0.f, s2 = 0.f;
     const int e1 = d.cell_ptr[k + 1];
-    for (int e = d.cell_ptr[k]; e < e1; ++e) {
+    int e = d.cell_ptr[k];
+    for (; e + 4 <= e1; e += 4) {   // four slots' loads in flight, summed in slot order
+      int slot[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) slot[u] = d.cell_slot[e + u];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int m = slot[u] / kS3;
+        const float w = d.W[slot[u]];
+        s0 += mf[3 * m] * w;
+        s1 += mf[3 * m + 1] * w;
+        s2 += mf[3 * m + 2] * w;
+      }
+    }
+    for (; e < e1; ++e) {
       const int slot = d.cell_slot[e];
       const int m = slot / kS3;
       const float w = d.W[slot];
@@ -254,7 +273,7 @@ void win_u_any(const StepCtx &c, const IB3DDev &d, const void *src, bool first, 
 
 int ib3d_phase(const StepCtx &c, const IB3DDev &d, const void *src, bool first, double *hist, hipStream_t s) {
   win_u_any(c, d, src, first, s);
-  const unsigned gm = (unsigned)((d.M + 255) / 256), gc = (unsigned)((d.cells + 255) / 256);
+  const unsigned gm = (unsigned)((d.M + 3) / 4), gc = (unsigned)((d.cells + 255) / 256);   // k_mark3d: 4 markers / block
   hipLaunchKernelGGL(k_mark3d, dim3(gm), dim3(256), 0, s, d, 0);
   for (int it = 1; it < d.n_iter; ++it) {
     hipLaunchKernelGGL(k_spread3d, dim3(gc), dim3(256), 0, s, d, (const float *)d.dF, d.acc, (double *)nullptr);
