@@ -149,11 +149,14 @@ def small_case(cls, **kw):
 @pytest.mark.parametrize("variant", [dict(), dict(collision="kbc", forcing="edm"), dict(collision="reg", forcing="edm"),
                                      dict(collision="bgk", forcing="guo", dof=(True, True), kernel="cosine_4pt"),
                                      dict(kernel="peskin_3pt", n_iter=1, inlet="fc_nebb", outlet="eq"),
-                                     dict(cx=12.0, n_markers=48), dict(cx=150.0, cy=8.0)],
+                                     dict(cx=12.0, n_markers=48), dict(cx=150.0, cy=8.0), dict(n_markers=600)],
                          ids=["mrt_guo", "kbc_edm", "reg_edm", "bgk_guo_2dof_cos", "p3_nebb", "window_on_inlet",
-                              "window_on_outlet_clamped"])
+                              "window_on_outlet_clamped", "dense_markers"])
 @pytest.mark.parametrize("fused", [True, False], ids=["fused", "separate"])
 def test_viv_matches_oracle(sim, variant, fused):
+    """dense_markers: 600 markers give 9600 stencil entries, more than the
+    segmented spread of the IB workgroup holds (8 per thread), so the binned
+    gather spreads them."""
     oc = small_case(S.VIV2D, **variant)
     pc = small_case(sim.VIVCase, **variant)
     steps = 40
