@@ -122,3 +122,40 @@ def test_batch_rejects_bad_input(sim):
         sim.VIVBatch([c], tile_y=96)
     with pytest.raises(ValueError):
         sim.VIVBatch([c, sweep_case(sim.VIVCase, *SHAPES[1], collision="mrt", forcing="guo")])
+
+
+def test_batch_split_mode_matches_oracle(sim):
+    """Windows of 84 x 84 cells (D = 64, 268 markers) need more LDS than two
+    bulk workgroups per CU leave: the batch then runs each step as an IB launch
+    for all cases followed by a bulk launch.  Both cases must match the oracle,
+    and a replayed graph must continue the run."""
+    steps = 10
+    v0 = np.array([0.0, 1e-3])
+    shapes = [(192, 512, 2.0, 5.0, 300.0), (160, 512, 4.0, 3.0, 600.0)]
+    pcs = [sweep_case(sim.VIVCase, *s, pad=10) for s in shapes]
+    ocs = [sweep_case(S.VIV2D, *s, pad=10) for s in shapes]
+    assert pcs[0].ib_size == 84
+    say("batch split mode")
+    b = sim.VIVBatch(pcs, hist_cap=2 * steps)
+    b.set_bodies(v=v0)
+    f0 = b.initial_f()
+    got = b.run(f0, steps)
+    b.check()
+    for k, (f, d, v, a, dh, hh) in enumerate(oracle_runs(ocs, steps, v0)):
+        assert_close(b.view(got, k), f, 5e-5, f"split case {k} f")
+        assert_close(b.hist_h[k][:steps], hh, 1e-4, f"split case {k} h")
+    # graph replays continue the clock: a 5-step graph replayed twice after the run
+    ref = sim.VIVBatch(pcs, hist_cap=2 * steps)
+    ref.set_bodies(v=v0)
+    ref_out = ref.run(f0, 2 * steps)
+    fa = got.clone()
+    fb = torch.empty_like(fa)
+    g_ab = b.graph(fa, fb, 5)
+    g_ba = b.graph(fb, fa, 5)
+    g_ab.launch()
+    g_ba.launch()
+    torch.cuda.synchronize()
+    b.check()
+    assert b.clock() == 2 * steps
+    for k in range(len(pcs)):
+        assert_close(b.view(fa, k), to_np(ref.view(ref_out, k)).astype(np.float64), 1e-5, f"split graph case {k}")

@@ -15,6 +15,10 @@
 // step) from device memory, so a captured HIP graph replays correctly: the
 // ready flags, tile counters, body slots and history rows all follow the
 // clock, which each run advances.
+// Split mode (IB windows whose LDS would leave room for only one workgroup
+// per CU, which halves the bulk tiles' occupancy): each step is two
+// launches, the IB workgroups of all cases (k_viv_batch_ib), then the bulk
+// tiles without LDS, which apply the force windows without waiting.
 #include <algorithm>
 #include <cstring>
 #include <type_traits>
@@ -37,14 +41,14 @@ constexpr int kCaseShift = 20;   // block map entry = case << 20 | tile index
 
 // Populations of one tile cell of case `bc`.  PHASE: COLLIDE reads the
 // caller's post-streaming state in place, FULL / STREAM pull and apply walls.
-template <class SRC, class DST, int COLL, int PHASE>
+template <class SRC, class DST, int COLL, int PHASE, bool SPLIT>
 __global__ __attribute__((amdgpu_flat_work_group_size(kTileCells, kTileCells), amdgpu_waves_per_eu(8, 8))) void
 k_viv_batch(const BatchCase *__restrict__ tab, const uint32_t *__restrict__ bmap, int n_cases,
             const typename SRC::T *__restrict__ src, typename DST::T *__restrict__ dst, int step,
             const int32_t *__restrict__ clock) {
   extern __shared__ float lds[];
   constexpr bool IB = PHASE != VSX_PHASE_STREAM;
-  const int b = blockIdx.x;
+  const int b = SPLIT ? (int)blockIdx.x + n_cases : (int)blockIdx.x;   // split: tiles only
   const int gstep = IB ? *clock + step : 0;
   const int slot = gstep & 1;
   const BatchCase &bc = b < n_cases ? tab[b] : tab[bmap[b - n_cases] >> kCaseShift];
@@ -64,8 +68,8 @@ k_viv_batch(const BatchCase *__restrict__ tab, const uint32_t *__restrict__ bmap
     org[1] = s_org[1];
     window_tiles(bc.T, org[0], org[1], bc.ib.win, wx0, wx1, wy0, wy1);
   }
-  if (b < n_cases) {
-    if constexpr (IB) {
+  if (!SPLIT && b < n_cases) {
+    if constexpr (IB && !SPLIT) {
       ib_block<SRC, PHASE == VSX_PHASE_FULL, 1, 2, 2, false, 1, false>(c, bc.ib, src + bc.f_off,
                                                              gstep < bc.hist_cap ? gstep : -1, gstep, 0,
                                                              bc.ib.body + 6 * slot, bc.ib.body + 6 * (slot ^ 1), lds);
@@ -93,13 +97,29 @@ k_viv_batch(const BatchCase *__restrict__ tab, const uint32_t *__restrict__ bmap
       else pull<D2Q9, SRC>(c, s_case, co[0], co[1], 0, f);
     }
   }
-  if constexpr (IB) {
+  if constexpr (IB && !SPLIT) {
     if (win) wait_ready(bc.ib, gstep + 1);
   }
   if (!valid) return;
   if constexpr (PHASE != VSX_PHASE_STREAM) finish<D2Q9, COLL>(c, cell, co, org, win, f);
   store_cell<D2Q9, DST>(c, d_case, cell, f);
 }
+
+// Split mode: the IB workgroup of every case (one workgroup per case).
+template <class SRC, bool PULL>
+__global__ __launch_bounds__(kIbThreads) void k_viv_batch_ib(const BatchCase *__restrict__ tab,
+                                                             const typename SRC::T *__restrict__ src, int step,
+                                                             const int32_t *__restrict__ clock) {
+  extern __shared__ float lds[];
+  const BatchCase &bc = tab[blockIdx.x];
+  const int gstep = *clock + step, slot = gstep & 1;
+  ib_block<SRC, PULL, 1, 4, 3, false, 2, true>(bc.c, bc.ib, src + bc.f_off, gstep < bc.hist_cap ? gstep : -1, gstep, 0,
+                                               bc.ib.body + 6 * slot, bc.ib.body + 6 * (slot ^ 1), lds);
+}
+
+// Dynamic LDS above which the IB workgroups run in their own launch (two
+// 1024-thread bulk workgroups per CU need each at most half of the 160 KB).
+constexpr int kBatchSplitLds = 76 * 1024;
 
 __global__ void k_clock_advance(int32_t *clock, int n) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *clock += n;
@@ -130,9 +150,19 @@ void launch_batch_c(const VsxVivBatch &b, const void *src, void *dst, int step, 
   const auto *tab = static_cast<const BatchCase *>(b.table);
   const auto *bmap = reinterpret_cast<const uint32_t *>(static_cast<const char *>(b.table) +
                                                         sizeof(BatchCase) * b.n_cases);
-  hipLaunchKernelGGL((k_viv_batch<SRC, DST, COLL, PHASE>), dim3((unsigned)b.n_blocks), dim3(kTileCells),
-                     (size_t)b.lds_bytes, s, tab, bmap, b.n_cases, static_cast<const typename SRC::T *>(src),
-                     static_cast<typename DST::T *>(dst), step, (const int32_t *)b.clock);
+  auto S = static_cast<const typename SRC::T *>(src);
+  auto D = static_cast<typename DST::T *>(dst);
+  const int32_t *clk = b.clock;
+  if (b.lds_bytes > kBatchSplitLds) {
+    if constexpr (PHASE != VSX_PHASE_STREAM)
+      hipLaunchKernelGGL((k_viv_batch_ib<SRC, PHASE == VSX_PHASE_FULL>), dim3((unsigned)b.n_cases), dim3(kIbThreads),
+                         (size_t)b.lds_bytes, s, tab, S, step, clk);
+    hipLaunchKernelGGL((k_viv_batch<SRC, DST, COLL, PHASE, true>), dim3((unsigned)(b.n_blocks - b.n_cases)),
+                       dim3(kTileCells), 0, s, tab, bmap, b.n_cases, S, D, step, clk);
+  } else {
+    hipLaunchKernelGGL((k_viv_batch<SRC, DST, COLL, PHASE, false>), dim3((unsigned)b.n_blocks), dim3(kTileCells),
+                       (size_t)b.lds_bytes, s, tab, bmap, b.n_cases, S, D, step, clk);
+  }
 }
 
 template <class ST, int PHASE>
