@@ -246,40 +246,46 @@ __device__ __forceinline__ int cell_entries(const uint16_t *bend, int k, int S) 
 }
 
 // Builds the entries (every thread calls it; ends with a barrier).  Returns
-// false when the binned markers have more entries than the workgroup holds
-// (16 n > kSpreadEPT nt); the caller then uses bin_spread.  tmp: uint16
-// scratch of 2 SS + 2 (entry offset of each active cell, entry count of each
-// cell); act: uint16 scratch for the active cells; fcell, lcell: first and
-// last cell of each thread's entries (written after the entries are built, so
-// they may overlay bend and act).  rec holds the weights and, after the
-// forces, the base cell of each binned marker (uint16).
-__device__ __forceinline__ bool seg_build(const float *rec, int M, const uint16_t *bend, int S, uint16_t *act,
-                                          uint16_t *tmp, uint16_t *fcell, uint16_t *lcell, int *s_wsum, SpreadSeg &sg) {
+// false (before any barrier, the same on every thread) when the binned markers
+// have more entries than the workgroup holds (16 n > kSpreadEPT nt) or than
+// the staging area takes; the caller then uses bin_spread.  tmp: uint16
+// scratch of 4 SS (first entry of each cell, then the staged entries);
+// fcell, lcell: first and last cell of each thread's entries (written after
+// the entries are built, so they may overlay bend).  rec holds the weights
+// and, after the forces, the base cell of each binned marker (uint16).
+// binned / pos / base: this thread's marker, its bin-order position and base
+// cell.  Order of the entries: cells ascending; within a cell runs a = 0..3,
+// markers in bin order, as bin_spread sums them.  Each marker writes its 16
+// entries at their ranks: cell k = base + a S + b gets from run a the
+// markers of bins k - a S - 3 .. k - a S, so the rank is the cell's first
+// entry + the entries of its runs 0..a-1 + (pos - first marker of run a).
+__device__ __forceinline__ bool seg_build(const float *rec, int M, const uint16_t *bend, int S, uint16_t *tmp,
+                                          uint16_t *fcell, uint16_t *lcell, int *s_wsum, SpreadSeg &sg, bool binned,
+                                          int pos, int base) {
   const int SS = S * S, tid = threadIdx.x, nt = blockDim.x;
   const uint16_t *binof = reinterpret_cast<const uint16_t *>(rec + 10 * M);
-  uint16_t *rptr = tmp, *ccount = tmp + SS + 2;
   const int T = 16 * (int)bend[SS - 1];
+  const int st0 = (SS + 8) & ~7;   // staged entries, 16-byte aligned
   sg.nthr = (T + kSpreadEPT - 1) / kSpreadEPT;
-  if (T > kSpreadEPT * nt) return false;   // uniform: every thread reads the same bend entry
-  // active cells in ascending order with their first entry: per-thread chunks
-  // of consecutive cells, one block scan of (entries << 16 | active cells)
+  if (T > kSpreadEPT * nt || st0 + kSpreadEPT * sg.nthr > 4 * SS) return false;
+  uint16_t *cstart = tmp, *stage = tmp + st0;
+  // first entry of every cell: per-thread chunks of consecutive cells, one
+  // block scan of the entry counts
   const int chunk = (SS + nt - 1) / nt, k0 = min(tid * chunk, SS), k1 = min(k0 + chunk, SS);
-  int mine = 0;
-  for (int k = k0; k < k1; ++k) {
+  auto cell_count = [&](int k) {
     int e = 0;
 #pragma unroll
     for (int a = 0; a < 4; ++a) e += bend[k - a * S] - bend[k - a * S - 4];
-    ccount[k] = (uint16_t)e;
-    mine += e ? (e << 16) + 1 : 0;
-  }
+    return e;
+  };
+  int mine = 0;
+  for (int k = k0; k < k1; ++k) mine += cell_count(k);
   int v = mine;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
     const int t = __shfl_up(v, o, 64);
     if ((tid & 63) >= o) v += t;
   }
-  __shared__ int s_nact;
-  __shared__ uint16_t s_tcell[kIbThreads];   // active cell holding each thread's first entry
   if ((tid & 63) == 63) s_wsum[tid >> 6] = v;
   __syncthreads();
   if (tid < 64) {   // exclusive scan of the wave totals by the first wavefront
@@ -292,77 +298,59 @@ __device__ __forceinline__ bool seg_build(const float *rec, int M, const uint16_
       if (tid >= o) y += t;
     }
     if (tid < nw) s_wsum[tid] = y - x;
-    if (tid == nw - 1) {
-      s_nact = y & 0xffff;
-      rptr[y & 0xffff] = (uint16_t)T;   // end of the last active cell
-    }
   }
   __syncthreads();
-  const int at = s_wsum[tid >> 6] + v - mine;
-  int q = at & 0xffff, r = at >> 16;
+  int r = s_wsum[tid >> 6] + v - mine;
   for (int k = k0; k < k1; ++k) {
-    const int e = ccount[k];
-    if (e) {
-      act[q] = (uint16_t)k;
-      rptr[q] = (uint16_t)r;
-      for (int i = (r + kSpreadEPT - 1) / kSpreadEPT; i * kSpreadEPT < r + e; ++i) s_tcell[i] = (uint16_t)q;
-      ++q;
-      r += e;
-    }
+    cstart[k] = (uint16_t)r;
+    r += cell_count(k);
   }
   __syncthreads();
-  // this thread's entries [g0, g0 + kSpreadEPT), walked with the run lengths
-  // and first markers of the current cell in registers; the cell advances when
-  // an entry reaches the next cell's first entry
-  const int g0 = tid * kSpreadEPT;
-  int cq = 0, k = 0, start = 0, next = 0x7fffffff, n[4] = {0, 0, 0, 0}, e0[4] = {0, 0, 0, 0};
-  auto load_cell = [&]() {
-    k = act[cq];
-    start = rptr[cq];
-    next = rptr[cq + 1];
+  // every binned marker writes its 16 entries at their ranks
+  if (binned) {
+    int e0b[4], pre[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int b = 0; b < 4; ++b) e0b[b] = bend[base + b - 4];
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
-      e0[a] = bend[k - a * S - 4];
-      n[a] = bend[k - a * S] - e0[a];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int k = base + a * S + b;
+        if (a > 0) pre[b] += bend[base + a * S + b] - bend[base + a * S + b - 4];
+        stage[cstart[k] + pre[b] + pos - e0b[b]] = (uint16_t)(pos | (a << 10) | (b << 12));
+      }
     }
-  };
-  if (g0 < T) {
-    cq = s_tcell[tid];
-    load_cell();
   }
-  int kfirst = 0;
+  __syncthreads();   // bend and cstart are dead from here
+  // this thread's entries [8 tid, 8 tid + 8): past the end, the last entry
+  // again with weight 0
+  const int g0 = tid * kSpreadEPT;
+  int kfirst = 0, klast = 0;
 #pragma unroll
   for (int t = 0; t < kSpreadEPT / 2; ++t) sg.eab[t] = 0;
+  if (g0 < T) {
+    const uint4 raw = *reinterpret_cast<const uint4 *>(stage + g0);
+    const uint32_t w4[4] = {raw.x, raw.y, raw.z, raw.w};
+    uint32_t last = 0;
 #pragma unroll
-  for (int t = 0; t < kSpreadEPT; ++t) {
-    sg.w[t] = 0.f;
-    if (g0 >= T) continue;   // no entries (the thread takes no part in the spread)
-    const int g = min(g0 + t, T - 1);   // past the end: the last entry again, with weight 0
-    if (g == next) {
-      ++cq;
-      load_cell();
+    for (int t = 0; t < kSpreadEPT; ++t) {
+      uint32_t v16 = (w4[t >> 1] >> (16 * (t & 1))) & 0xffffu;
+      v16 = g0 + t < T ? v16 : last;
+      last = v16;
+      sg.eab[t >> 1] |= v16 << (16 * (t & 1));
+      const int e = v16 & 1023, a = (v16 >> 10) & 3, b = (v16 >> 12) & 3;
+      sg.w[t] = g0 + t < T ? rec[a * M + e] * rec[(4 + b) * M + e] : 0.f;
+      const int k = (int)binof[e] + a * S + b;
+      if (t == 0) kfirst = k;
+      klast = k;
     }
-    int o = g - start, a = 0;
+  } else {
 #pragma unroll
-    for (int aa = 0; aa < 3; ++aa) {
-      const bool past = a == aa && o >= n[aa];
-      o -= past ? n[aa] : 0;
-      a += past ? 1 : 0;
-    }
-    const int e = (a == 0 ? e0[0] : a == 1 ? e0[1] : a == 2 ? e0[2] : e0[3]) + o;
-    // stencil point (a, b) of marker e is cell k: b = k - a S - (its base cell)
-    const int b = k - a * S - (int)binof[e];
-    const uint32_t v = (uint32_t)e | ((uint32_t)a << 10) | ((uint32_t)b << 12);
-    sg.eab[t >> 1] |= v << (16 * (t & 1));
-    const float wxa = rec[a * M + e];
-    const float wy0 = rec[4 * M + e], wy1 = rec[5 * M + e], wy2 = rec[6 * M + e], wy3 = rec[7 * M + e];
-    sg.w[t] = g0 + t < T ? wxa * (b == 0 ? wy0 : b == 1 ? wy1 : b == 2 ? wy2 : wy3) : 0.f;
-    if (t == 0) kfirst = k;
+    for (int t = 0; t < kSpreadEPT; ++t) sg.w[t] = 0.f;
   }
-  __syncthreads();   // bend, act and rptr are dead from here
   if (tid < sg.nthr) {
     fcell[tid] = (uint16_t)kfirst;
-    lcell[tid] = (uint16_t)k;
+    lcell[tid] = (uint16_t)klast;
   }
   __syncthreads();
   return true;
@@ -709,7 +697,8 @@ __device__ __forceinline__ void ib_block(const StepCtx &c, const VsxIB2D &ib, co
       }
     }
     if constexpr (SEG)
-      use_seg = seg_build(rec, M, bend, S, act, reinterpret_cast<uint16_t *>(lds), fcell, lcell, s_wsum, sg);
+      use_seg = seg_build(rec, M, bend, S, reinterpret_cast<uint16_t *>(lds), fcell, lcell, s_wsum, sg, binned, my_pos,
+                          base_idx[0]);
     if (!use_seg) n_act = active_cells(bend, act, S, &s_nact);   // ends with a barrier
   }
 
