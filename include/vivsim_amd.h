@@ -84,6 +84,8 @@ typedef struct VsxLbm {
   VsxWall walls[6];
   const float *g;          /* optional full body-force field [D][NX][NY(][NZ)] (device)        */
   float g_uniform[3];      /* uniform body force added to g                                     */
+  float store_u[3];        /* 16-bit storages keep f_i - feq_i(1, store_u) (0: f_i - w_i); a
+                              reference velocity near the flow's keeps the stored numbers small */
   int32_t reserved;
 } VsxLbm;
 

@@ -42,7 +42,8 @@ class VsxLbm(ctypes.Structure):
     _fields_ = [("nx", c_int32), ("ny", c_int32), ("nz", c_int32),
                 ("collision", c_int32), ("forcing", c_int32), ("storage", c_int32),
                 ("omega", c_float), ("mrt_rates", c_float * 19), ("n_walls", c_int32),
-                ("walls", VsxWall * 6), ("g", c_void_p), ("g_uniform", c_float * 3), ("reserved", c_int32)]
+                ("walls", VsxWall * 6), ("g", c_void_p), ("g_uniform", c_float * 3), ("store_u", c_float * 3),
+                ("reserved", c_int32)]
 
 
 class VsxIB2D(ctypes.Structure):

@@ -50,6 +50,7 @@ struct StepCtx {
   // -2: several walls at LOC), at[LOC] = that wall.
   int32_t order[6];
   VsxWall at[6];
+  float shift[19];         // 16-bit storage: stored value = f_i - shift[i] = f_i - feq_i(1, p.store_u)
 };
 
 // The kernel's StepCtx argument, addressed in the kernarg segment (it is always
@@ -79,7 +80,7 @@ __device__ __forceinline__ void pull(const StepCtx &c, const typename SRC::T *__
     const int xs = wrap(x - L::cx[i], c.dims[0]);
     const int ys = wrap(y - L::cy[i], c.dims[1]);
     const int zs = L::D == 3 ? wrap(z - L::cz[i], c.dims[2]) : 0;
-    f[i] = SRC::ld(src + (size_t)i * c.n, cell_of<L>(c, xs, ys, zs), L::w[i]);
+    f[i] = SRC::ld(src + (size_t)i * c.n, cell_of<L>(c, xs, ys, zs), c.shift[i]);
   }
 }
 
@@ -87,7 +88,7 @@ template <class L, class SRC>
 __device__ __forceinline__ void local(const StepCtx &c, const typename SRC::T *__restrict__ src, uint32_t cell,
                                       float (&f)[L::Q]) {
 #pragma unroll
-  for (int i = 0; i < L::Q; ++i) f[i] = SRC::ld(src + (size_t)i * c.n, cell, L::w[i]);
+  for (int i = 0; i < L::Q; ++i) f[i] = SRC::ld(src + (size_t)i * c.n, cell, c.shift[i]);
 }
 
 // component a of a coordinate triple, by selects (no dynamic register indexing)
@@ -344,7 +345,7 @@ template <class L, class DST>
 __device__ __forceinline__ void store_cell(const StepCtx &c, typename DST::T *__restrict__ dst, uint32_t cell,
                                            const float (&f)[L::Q]) {
 #pragma unroll
-  for (int i = 0; i < L::Q; ++i) DST::st(dst + (size_t)i * c.n, cell, f[i], L::w[i]);
+  for (int i = 0; i < L::Q; ++i) DST::st(dst + (size_t)i * c.n, cell, f[i], c.shift[i]);
 }
 
 template <class L, int COLL>

@@ -42,6 +42,14 @@ StepCtx make_ctx(const VsxLbm &p, Window win) {
     c.order[l] = c.order[l] == -1 ? k : -2;
     c.at[l] = p.walls[k];
   }
+  // shift of the 16-bit storages: feq_i(1, store_u) (lbm/basic.py:114-135 at rho = 1)
+  const float u0 = p.store_u[0], u1 = p.store_u[1], u2 = L::D == 3 ? p.store_u[2] : 0.f;
+  const float usq = u0 * u0 + u1 * u1 + u2 * u2;
+  for (int i = 0; i < 19; ++i) c.shift[i] = 0.f;
+  for (int i = 0; i < L::Q; ++i) {
+    const float cu = L::cx[i] * u0 + L::cy[i] * u1 + L::cz[i] * u2;
+    c.shift[i] = L::w[i] * (1.f + 3.f * cu + 4.5f * cu * cu - 1.5f * usq);
+  }
   return c;
 }
 

@@ -37,7 +37,10 @@ class LBM:
     """A fused LBM step configuration for a grid of ``shape`` (2 or 3 axes)."""
 
     def __init__(self, shape, omega, collision="bgk", walls=(), forcing=None, g=None, g_uniform=None,
-                 storage="f32", mrt_rates=None):
+                 storage="f32", mrt_rates=None, store_u=None):
+        """store_u: reference velocity of the 16-bit storages, which keep
+        f_i - feq_i(1, store_u) (default 0: f_i - w_i); a velocity near the
+        flow's keeps the stored numbers small and their rounding error too."""
         shape = tuple(int(s) for s in shape)
         require(len(shape) in (2, 3), "shape must have 2 or 3 axes")
         self.shape, self.dim = shape, len(shape)
@@ -72,6 +75,8 @@ class LBM:
             p.g = self._g.data_ptr()
         for a, v in enumerate(g_uniform or ()):
             p.g_uniform[a] = float(v)
+        for a, v in enumerate(store_u or ()):
+            p.store_u[a] = float(v)
         self.p = p
         self._work = None
 
@@ -233,13 +238,14 @@ class _Body2D:
 
 
 class VIV2D:
-    """Device state and fused run of a ``VIVCase``."""
+    """Device state and fused run of a ``VIVCase``.  16-bit storages keep the
+    populations relative to the equilibrium of the inflow (U0, 0)."""
 
     def __init__(self, case: VIVCase, storage="f32", fused=True):
         self.case = case
         dev = L.device()
         self.lbm = LBM((case.nx, case.ny), case.omega, collision=case.collision, walls=case.walls(),
-                       forcing=case.forcing, storage=storage)
+                       forcing=case.forcing, storage=storage, store_u=(case.U0, 0.0))
         self._b = _Body2D(case, dev)
         self.body, self.win_origin, self.g_win = self._b.body, self._b.win_origin, self._b.g_win
         self.t_total = 0
@@ -359,7 +365,7 @@ class Sphere3D:
         self.case = case
         dev = L.device()
         self.lbm = LBM((case.nx, case.ny, case.nz), case.omega, collision=case.collision, walls=case.walls(),
-                       forcing=case.forcing, storage=storage)
+                       forcing=case.forcing, storage=storage, store_u=(case.U0, 0.0, 0.0))
         S = case.ib_size
         org = case.ib_origin
         # stencil of the fixed markers relative to the window origin (flow_past_sphere.py:114-121)
@@ -450,7 +456,7 @@ class VIVBatch:
         dev = L.device()
         self.cases, self.storage, self.hist_cap = cases, storage, int(hist_cap)
         self.lbms = [LBM((c.nx, c.ny), c.omega, collision=c.collision, walls=c.walls(), forcing=c.forcing,
-                         storage=storage) for c in cases]
+                         storage=storage, store_u=(c.U0, 0.0)) for c in cases]
         self.bodies = [_Body2D(c, dev) for c in cases]
         self.offsets, off = [], 0
         for c in cases:
