@@ -13,7 +13,9 @@ Also reported on the same line:
   roofline      : the fused step kernel's achieved algorithmic bandwidth
                   (72 B per cell in FP32 / 36 B in 16-bit storage) from HIP events
                   recorded around every launch in the timed region, vs 8 TB/s
-  variants      : FP16 / BF16 shifted-storage GLUPS of the same step
+  variants      : FP16 / BF16 / E3M12 shifted-storage GLUPS of the same step,
+                  the step as two launches, 4096^2 in FP32 and 16-bit storage, the
+                  3D sphere, a 64-case batch and the adjoint / forward ratio
   cpu_baseline  : the oracle (NumPy FP32 restatement of the reference step),
                   single-threaded on a bounded sample of the same workload
 """
@@ -189,7 +191,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--nx", type=int, default=1024)
     ap.add_argument("--ny", type=int, default=1024)
-    ap.add_argument("--storage", default="f32", choices=["f32", "f16s", "bf16s"])
+    ap.add_argument("--storage", default="f32", choices=["f32", "f16s", "bf16s", "e3m12s"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-max-steps", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
@@ -222,7 +224,7 @@ def main():
     main_run = run_gpu(args, args.storage, args.steps, args.warmup)
     variants = {}
     if not args.no_variants:
-        for st in ("f32", "f16s", "bf16s"):
+        for st in ("f32", "f16s", "bf16s", "e3m12s"):
             if st != args.storage:
                 r = run_gpu(args, st, args.steps, args.warmup)
                 variants[st] = dict(glups=round(r["glups"], 3), ms_per_step=round(r["ms_per_step"], 5),
@@ -237,6 +239,14 @@ def main():
         variants["grid_4096"] = dict(glups=round(r["glups"], 3), ms_per_step=round(r["ms_per_step"], 5),
                                      kernel_us=round(r["kernel_ms"] * 1e3, 3), achieved_gbs=round(r["achieved_gbs"], 1),
                                      frac=round(r["achieved_gbs"] / HBM_PEAK_GBS, 4))
+        # 16-bit storage where the streaming, not the IB workgroup, bounds the step
+        for st in ("f16s", "e3m12s"):
+            r16 = run_gpu(big, st, 200, 20)
+            variants[f"grid_4096_{st}"] = dict(glups=round(r16["glups"], 3), ms_per_step=round(r16["ms_per_step"], 5),
+                                               kernel_us=round(r16["kernel_ms"] * 1e3, 3),
+                                               achieved_gbs=round(r16["achieved_gbs"], 1),
+                                               frac=round(r16["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                                               speedup_vs_f32=round(r16["glups"] / r["glups"], 3))
         variants["sphere3d"] = run_sphere(100)
         variants["batch64"] = run_batch(400)
         variants["adjoint"] = run_adjoint(args.adjoint_steps)

@@ -53,8 +53,10 @@ enum { VSX_FORCE_NONE = 0, VSX_FORCE_GUO = 1, VSX_FORCE_EDM = 2 };
 enum { VSX_KERNEL_PESKIN_2PT = 0, VSX_KERNEL_PESKIN_3PT = 1, VSX_KERNEL_PESKIN_4PT = 2,
        VSX_KERNEL_COSINE_4PT = 3 };
 
-/* population storage of the fused steps: FP32, or FP16 / BF16 deviations f - w_i */
-enum { VSX_STORE_F32 = 0, VSX_STORE_F16S = 1, VSX_STORE_BF16S = 2 };
+/* population storage of the fused steps: FP32, or 16-bit deviations f_i - feq_i(1, store_u) as
+ * FP16, BF16 or E3M12 (sign, 15-bit magnitude code: 12-13 significant bits, |deviation| < 0.249,
+ * larger values saturate) */
+enum { VSX_STORE_F32 = 0, VSX_STORE_F16S = 1, VSX_STORE_BF16S = 2, VSX_STORE_E3M12S = 3 };
 
 /* phases of the fused pull-scheme step (the stored state is post-collision) */
 enum { VSX_PHASE_COLLIDE = 0,   /* read f locally (post-stream state), collide, write          */

@@ -2,8 +2,9 @@
 the VIV case scaled to 256 x 256 (D = 20, 256 markers, MRT + Guo, Re = 100,
 U_r = 5, mass ratio 2), 1000 steps.  Errors are stated for the lift
 coefficient history, the transverse displacement history and the final
-velocity field, for FP32 and the 16-bit shifted storages (FP32 arithmetic),
-which keep f_i - feq_i(1, (U0, 0)); "f16s_rest" keeps f_i - w_i instead.
+velocity field, for FP32 and the 16-bit shifted storages (FP32 arithmetic: FP16, BF16 and
+the E3M12 code), which keep f_i - feq_i(1, (U0, 0)); "f16s_rest" keeps
+f_i - w_i instead.
 The measured values are written to gpurun_out/accuracy.json when that
 folder exists."""
 
@@ -42,7 +43,7 @@ def test_storage_accuracy(gpu_lib, fp64):
     c, dh, hh, u64 = fp64
     cl = lambda h: h[:, 1] / (0.5 * c.U0 ** 2 * c.D)
     out = {}
-    for name in ("f32", "f16s", "bf16s", "f16s_rest"):
+    for name in ("f32", "f16s", "bf16s", "e3m12s", "f16s_rest"):
         storage = name.split("_")[0]
         pc = sim.VIVCase(**GEOM)
         solver = sim.VIV2D(pc, storage=storage)
@@ -63,3 +64,4 @@ def test_storage_accuracy(gpu_lib, fp64):
             json.dump(out, fh, indent=1)
     assert out["f32"]["lift_coefficient"] < 1e-3 and out["f32"]["velocity"] < 1e-4
     assert out["f16s"]["velocity"] < 2e-2 and out["bf16s"]["velocity"] < 2e-1
+    assert out["e3m12s"]["lift_coefficient"] < 1e-3 and out["e3m12s"]["velocity"] < 1e-3

@@ -169,7 +169,7 @@ def test_fused3d_forcing(sim, collision, forcing):
     assert_close(got, ref, 3e-5, f"{collision}+{forcing}")
 
 
-@pytest.mark.parametrize("storage,tol", [("f16s", 3e-3), ("bf16s", 3e-2)])
+@pytest.mark.parametrize("storage,tol", [("f16s", 3e-3), ("bf16s", 3e-2), ("e3m12s", 1.5e-3)])
 def test_fused3d_half(sim, storage, tol):
     sh, n, om = (16, 12, 10), 12, 1.8
     f0 = field(sh)

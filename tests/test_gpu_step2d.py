@@ -126,7 +126,7 @@ def test_cavity_1000_steps(sim):
     say(f"cavity rel err f {e:.2e}")
 
 
-@pytest.mark.parametrize("storage,tol", [("f16s", 2e-3), ("bf16s", 2e-2)])
+@pytest.mark.parametrize("storage,tol", [("f16s", 2e-3), ("bf16s", 2e-2), ("e3m12s", 1e-3)])
 def test_half_storage(sim, storage, tol):
     nx, ny, n, om = 40, 32, 30, 1.7
     f0 = rand_state(nx, ny, u0=0.03)
@@ -135,6 +135,44 @@ def test_half_storage(sim, storage, tol):
     ref = run_oracle(f0, n, om, walls, "mrt", None, None)
     e = assert_close(got, ref, tol, storage)
     say(f"{storage}: rel err {e:.2e}")
+
+
+def test_e3m12_codes_match_model(sim):
+    """The device E3M12 conversions against tests/storage_model.py, bit for bit.
+    Decode: a STREAM pass on a periodic grid writes decode(code) + s_i pulled
+    along c_i (s_i = w_i here, read back from an all-zero code field).
+    Encode: a BGK COLLIDE pass with omega = 1e-30 leaves f unchanged in float32
+    and stores encode(f - s_i); the deviations span every block of the code
+    and values beyond its range (saturation)."""
+    from tests import storage_model as SM
+    nx, ny = 48, 40
+    lb = sim.LBM((nx, ny), 1e-30, collision="bgk", storage="e3m12s")
+    wa, wb = lb.work()
+    out = torch.empty((9, nx, ny), dtype=torch.float32, device="cuda")
+    wa.zero_()
+    lb.step(wa, out, "stream", dst_f32=True)
+    s = to_np(out)[:, 0, 0].astype(np.float32)
+    assert np.all(to_np(out) == s[:, None, None])
+
+    codes = rng.integers(0, 0x10000, size=(9, nx, ny)).astype(np.uint16)
+    wa.copy_(torch.from_numpy(codes.view(np.int16).reshape(-1)).to("cuda"))
+    lb.step(wa, out, "stream", dst_f32=True)
+    ref = O.streaming(SM.decode(codes) + s[:, None, None])
+    assert ref.dtype == np.float32 and np.array_equal(to_np(out).view(np.uint32), ref.view(np.uint32))
+
+    n = 9 * nx * ny
+    x = np.concatenate([rng.uniform(-0.26, 0.26, n // 2),
+                        np.sign(rng.standard_normal(n // 4)) * 2.0 ** rng.uniform(-30, -2, n // 4),
+                        SM.magnitudes()[rng.integers(0, 0x7fff, n - n // 2 - n // 4)]])
+    x = rng.permutation(x).reshape(9, nx, ny)
+    x[0] = -x[1:].sum(0)   # density 1: the (unused) equilibrium stays finite
+    f = (x + s[:, None, None]).astype(np.float32)
+    lb.step(dev(f), wb, "collide", src_f32=True)
+    want = SM.encode(f - s[:, None, None])
+    got = to_np(wb).astype(np.int16).view(np.uint16).reshape(9, nx, ny)
+    bad = np.count_nonzero(got != want)
+    say(f"e3m12: {codes.size} decodes, {want.size} encodes, {bad} mismatches")
+    assert bad == 0
 
 
 # ------------------------------------------------------------------ VIV (IB-LBM)

@@ -26,7 +26,7 @@ MODE = {"plain": 0, "velocity": 1, "pressure": 2, "force_corrected": 3}
 COLLISION = {"bgk": 0, "mrt": 1, "kbc": 2, "reg": 3}
 FORCING = {None: 0, "none": 0, "guo": 1, "edm": 2}
 KERNEL = {"peskin_2pt": 0, "peskin_3pt": 1, "peskin_4pt": 2, "cosine_4pt": 3}
-STORE = {"f32": 0, "f16s": 1, "bf16s": 2}
+STORE = {"f32": 0, "f16s": 1, "bf16s": 2, "e3m12s": 3}
 PHASE = {"collide": 0, "full": 1, "stream": 2}
 
 

@@ -187,7 +187,8 @@ void launch_batch(const VsxVivBatch &b, const void *src, void *dst, int step, hi
   switch (b.storage) {
     case VSX_STORE_F32: return launch_batch_p<Store<VSX_STORE_F32>, PHASE>(b, src, dst, step, s);
     case VSX_STORE_F16S: return launch_batch_p<Store<VSX_STORE_F16S>, PHASE>(b, src, dst, step, s);
-    default: return launch_batch_p<Store<VSX_STORE_BF16S>, PHASE>(b, src, dst, step, s);
+    case VSX_STORE_BF16S: return launch_batch_p<Store<VSX_STORE_BF16S>, PHASE>(b, src, dst, step, s);
+    default: return launch_batch_p<Store<VSX_STORE_E3M12S>, PHASE>(b, src, dst, step, s);
   }
 }
 

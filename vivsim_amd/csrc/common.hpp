@@ -45,10 +45,11 @@ inline unsigned grid_for(int64_t n, int block) {
 }
 
 // ------------------------------------------------------------------ storage
-// FP32 populations, or FP16 / BF16 "shifted" storage of the deviation f_i - w_i
-// (all arithmetic stays FP32).  Storing the deviation from the rest
-// equilibrium keeps the stored magnitude ~1e-2 instead of ~1e-1, which is
-// what makes 16-bit storage usable (Lehmann et al., 2022, "FP16S").
+// FP32 populations, or 16-bit "shifted" storage of the deviation f_i - s_i
+// from a fixed equilibrium s_i = feq_i(1, u_s) (all arithmetic stays FP32;
+// `w` below is s_i).  Storing the deviation keeps the stored magnitude ~1e-2
+// instead of ~1e-1, which is what makes 16-bit storage usable (Lehmann et
+// al., 2022, "FP16S").
 template <int S>
 struct Store;
 
@@ -79,6 +80,31 @@ struct Store<VSX_STORE_BF16S> {
   using T = uint16_t;
   __device__ static inline float ld(const T *p, uint32_t i, float w) { return bf16_to_f32(p[i]) + w; }
   __device__ static inline void st(T *p, uint32_t i, float v, float w) { p[i] = f32_to_bf16(v - w); }
+};
+
+// E3M12: 16 bits = sign | 15-bit magnitude code r, decoded as
+//   |x| = bits_to_float((r << 11) + (117 << 23)) - 2^-10,
+// i.e. 8 binades of 4096 steps each: [0, 2^-10) in steps of 2^-22, then
+// [2^(k-10) - 2^-10, 2^(k-9) - 2^-10) in steps of 2^(k-22), k = 1..7.  Against
+// FP16 (11 significant bits) this keeps 12-13 significant bits over the range
+// of shifted populations, |x| < 2^-2 - 2^-10; larger magnitudes (and NaN)
+// saturate to the largest code.
+__device__ inline float e3m12_to_f32(uint32_t h) {
+  const float m = __uint_as_float(((h & 0x7fffu) << 11) + (117u << 23)) - 0x1p-10f;
+  return __uint_as_float(__float_as_uint(m) | ((h & 0x8000u) << 16));
+}
+__device__ inline uint16_t f32_to_e3m12(float v) {
+  const uint32_t t = __float_as_uint(fabsf(v) + 0x1p-10f);   // >= 2^-10, so the rebias below cannot wrap
+  uint32_t r = (t + 0x3ffu + ((t >> 11) & 1u) - (117u << 23)) >> 11;   // round to nearest even
+  r = r < 0x7fffu ? r : 0x7fffu;
+  return (uint16_t)(r | ((__float_as_uint(v) >> 16) & 0x8000u));
+}
+
+template <>
+struct Store<VSX_STORE_E3M12S> {
+  using T = uint16_t;
+  __device__ static inline float ld(const T *p, uint32_t i, float w) { return e3m12_to_f32(p[i]) + w; }
+  __device__ static inline void st(T *p, uint32_t i, float v, float w) { p[i] = f32_to_e3m12(v - w); }
 };
 
 inline size_t store_bytes(int s) { return s == VSX_STORE_F32 ? 4 : 2; }

@@ -267,7 +267,8 @@ void win_u_any(const StepCtx &c, const IB3DDev &d, const void *src, bool first, 
   switch (c.p.storage) {
     case VSX_STORE_F32: return launch_win_u<Store<VSX_STORE_F32>, true>(c, d, src, s);
     case VSX_STORE_F16S: return launch_win_u<Store<VSX_STORE_F16S>, true>(c, d, src, s);
-    default: return launch_win_u<Store<VSX_STORE_BF16S>, true>(c, d, src, s);
+    case VSX_STORE_BF16S: return launch_win_u<Store<VSX_STORE_BF16S>, true>(c, d, src, s);
+    default: return launch_win_u<Store<VSX_STORE_E3M12S>, true>(c, d, src, s);
   }
 }
 

@@ -104,7 +104,8 @@ int viv_ib_any(const StepCtx &c, const VsxIB2D &ib, const void *src, bool first,
   switch (c.p.storage) {
     case VSX_STORE_F32: return launch_viv_ib<Store<VSX_STORE_F32>, true>(c, ib, src, step, s);
     case VSX_STORE_F16S: return launch_viv_ib<Store<VSX_STORE_F16S>, true>(c, ib, src, step, s);
-    default: return launch_viv_ib<Store<VSX_STORE_BF16S>, true>(c, ib, src, step, s);
+    case VSX_STORE_BF16S: return launch_viv_ib<Store<VSX_STORE_BF16S>, true>(c, ib, src, step, s);
+    default: return launch_viv_ib<Store<VSX_STORE_E3M12S>, true>(c, ib, src, step, s);
   }
 }
 
@@ -144,7 +145,8 @@ int fused_any(const StepCtx &c, const VsxIB2D &ib, const void *src, void *dst, i
   switch (c.p.storage) {
     case VSX_STORE_F32: return launch_fused<Store<VSX_STORE_F32>>(c, ib, src, dst, step, s);
     case VSX_STORE_F16S: return launch_fused<Store<VSX_STORE_F16S>>(c, ib, src, dst, step, s);
-    default: return launch_fused<Store<VSX_STORE_BF16S>>(c, ib, src, dst, step, s);
+    case VSX_STORE_BF16S: return launch_fused<Store<VSX_STORE_BF16S>>(c, ib, src, dst, step, s);
+    default: return launch_fused<Store<VSX_STORE_E3M12S>>(c, ib, src, dst, step, s);
   }
 }
 

@@ -13,7 +13,7 @@ int validate_lbm(const VsxLbm *p) {
   if ((int64_t)p->nx * p->ny * p->nz >= ((int64_t)1 << 31)) return fail(VSX_ERR_VALUE, "grid too large (>= 2^31 cells)");
   if (p->collision < VSX_COLL_BGK || p->collision > VSX_COLL_REG) return fail(VSX_ERR_VALUE, "unknown collision %d", p->collision);
   if (p->forcing < VSX_FORCE_NONE || p->forcing > VSX_FORCE_EDM) return fail(VSX_ERR_VALUE, "unknown forcing %d", p->forcing);
-  if (p->storage < VSX_STORE_F32 || p->storage > VSX_STORE_BF16S) return fail(VSX_ERR_VALUE, "unknown storage %d", p->storage);
+  if (p->storage < VSX_STORE_F32 || p->storage > VSX_STORE_E3M12S) return fail(VSX_ERR_VALUE, "unknown storage %d", p->storage);
   if (p->n_walls < 0 || p->n_walls > L::n_walls) return fail(VSX_ERR_VALUE, "bad wall count %d", p->n_walls);
   for (int k = 0; k < p->n_walls; ++k) {
     const VsxWall &w = p->walls[k];
@@ -128,7 +128,8 @@ int step_any(const StepCtx &c, const void *src, bool src_f32, void *dst, bool ds
   switch (c.p.storage) {
     case VSX_STORE_F32: return phase_dispatch<L, Store<VSX_STORE_F32>>(c, src, true, dst, true, phase, s);
     case VSX_STORE_F16S: return phase_dispatch<L, Store<VSX_STORE_F16S>>(c, src, src_f32, dst, dst_f32, phase, s);
-    default: return phase_dispatch<L, Store<VSX_STORE_BF16S>>(c, src, src_f32, dst, dst_f32, phase, s);
+    case VSX_STORE_BF16S: return phase_dispatch<L, Store<VSX_STORE_BF16S>>(c, src, src_f32, dst, dst_f32, phase, s);
+    default: return phase_dispatch<L, Store<VSX_STORE_E3M12S>>(c, src, src_f32, dst, dst_f32, phase, s);
   }
 }
 

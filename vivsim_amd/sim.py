@@ -27,7 +27,7 @@ import torch
 from . import _lib as L
 from ._util import WallBuilder, f32, numel, require, wall_nodes_shape
 
-_STORE_DTYPE = {"f32": torch.float32, "f16s": torch.float16, "bf16s": torch.bfloat16}
+_STORE_DTYPE = {"f32": torch.float32, "f16s": torch.float16, "bf16s": torch.bfloat16, "e3m12s": torch.int16}
 
 MRT_RATES_2D = lambda om: [0, 1.4, 1.4, 0, 1.2, 0, 1.2, om, om]                          # lbm/collision/mrt.py:44
 MRT_RATES_3D = lambda om: [0, 0, 0, 0, 1.1] + [om] * 5 + [1.2] * 6 + [1.4] * 3          # lbm3d/collision/mrt.py:92-113
