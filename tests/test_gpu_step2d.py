@@ -213,6 +213,45 @@ def test_viv_matches_oracle(sim, variant, fused):
     assert np.allclose(d_, d, rtol=1e-4, atol=1e-12)
 
 
+@pytest.mark.parametrize("storage,tol", [("f16s", 3e-3), ("bf16s", 3e-2), ("e3m12s", 1.5e-3)])
+@pytest.mark.parametrize("variant", [dict(), dict(collision="bgk", nx=168, ny=100), dict(collision="reg", forcing="edm")],
+                         ids=["mrt_guo", "bgk_odd_tiles", "reg_edm"])
+def test_viv_16bit_matches_oracle(sim, variant, storage, tol):
+    """The fused VIV step in 16-bit storage on small grids (one tile per bulk
+    block); the two-tile blocks of large grids are covered below."""
+    oc = small_case(S.VIV2D, **variant)
+    pc = small_case(sim.VIVCase, **variant)
+    steps = 40
+    f0 = oc.initial_f(np.float64)
+    v0 = np.array([0.0, 1e-3])
+    solver = sim.VIV2D(pc, storage=storage)
+    solver.set_body(v=v0)
+    say(f"viv16 {variant} {storage}")
+    got, hd, hh = solver.run(dev(f0), steps)
+    solver.check()
+    f, d, v, a, dh, hhist = S.viv_run(oc, f0, steps, v=v0)
+    assert_close(got, f, tol, f"viv {storage} f")
+
+
+@pytest.mark.parametrize("storage,tol", [("f16s", 3e-3), ("e3m12s", 1.5e-3)])
+def test_viv_16bit_pair_blocks_match_oracle(sim, storage, tol):
+    """Grids of at least 4096 tiles run the 16-bit bulk blocks on two tiles
+    each: 2052 x 2000 (4104 tiles of 4 x 256; a partial last tile row, the wall
+    columns and a lone last tile take the one-cell path), 3 steps."""
+    kw = dict(nx=2052, ny=2000, cx=256.0, cy=1000.0)
+    oc, pc = S.VIV2D(**kw), sim.VIVCase(**kw)
+    steps = 3
+    f0 = oc.initial_f(np.float64)
+    v0 = np.array([0.0, 1e-3])
+    solver = sim.VIV2D(pc, storage=storage)
+    solver.set_body(v=v0)
+    say(f"viv16 pair blocks {storage}")
+    got, hd, hh = solver.run(dev(f0), steps)
+    solver.check()
+    f, d, v, a, dh, hhist = S.viv_run(oc, f0, steps, v=v0)
+    assert_close(got, f, tol, f"viv {storage} f (pair blocks)")
+
+
 def test_viv_split_runs(sim):
     """Runs of 1 + 14 + 25 steps continue the body state and the step counter
     exactly as one run of 40 steps (the oracle's 40-step run)."""

@@ -56,6 +56,7 @@ struct Store;
 template <>
 struct Store<VSX_STORE_F32> {
   using T = float;
+  __device__ static inline float cvt(T r, float) { return r; }
   __device__ static inline float ld(const T *p, uint32_t i, float) { return p[i]; }
   __device__ static inline void st(T *p, uint32_t i, float v, float) { p[i] = v; }
 };
@@ -63,6 +64,7 @@ struct Store<VSX_STORE_F32> {
 template <>
 struct Store<VSX_STORE_F16S> {
   using T = _Float16;
+  __device__ static inline float cvt(T r, float w) { return (float)r + w; }
   __device__ static inline float ld(const T *p, uint32_t i, float w) { return (float)p[i] + w; }
   __device__ static inline void st(T *p, uint32_t i, float v, float w) { p[i] = (_Float16)(v - w); }
 };
@@ -78,6 +80,7 @@ __device__ inline uint16_t f32_to_bf16(float v) {
 template <>
 struct Store<VSX_STORE_BF16S> {
   using T = uint16_t;
+  __device__ static inline float cvt(T r, float w) { return bf16_to_f32(r) + w; }
   __device__ static inline float ld(const T *p, uint32_t i, float w) { return bf16_to_f32(p[i]) + w; }
   __device__ static inline void st(T *p, uint32_t i, float v, float w) { p[i] = f32_to_bf16(v - w); }
 };
@@ -103,6 +106,7 @@ __device__ inline uint16_t f32_to_e3m12(float v) {
 template <>
 struct Store<VSX_STORE_E3M12S> {
   using T = uint16_t;
+  __device__ static inline float cvt(T r, float w) { return e3m12_to_f32(r) + w; }
   __device__ static inline float ld(const T *p, uint32_t i, float w) { return e3m12_to_f32(p[i]) + w; }
   __device__ static inline void st(T *p, uint32_t i, float v, float w) { p[i] = f32_to_e3m12(v - w); }
 };

@@ -357,6 +357,16 @@ __device__ __forceinline__ void finish(const StepCtx &c, uint32_t cell, const in
   collide_force<L, COLL>(f, rho, u, g, c.p.forcing, c.p.omega, c.p.mrt_rates);
 }
 
+// finish() of a cell that carries no body force (no g field, no uniform g, not
+// in the IB window): the forced branches are compiled out.
+template <class L, int COLL>
+__device__ __forceinline__ void finish_unforced(const StepCtx &c, float (&f)[L::Q]) {
+  float rho, u[3];
+  macroscopic<L>(f, rho, u);
+  const float g[3] = {0.f, 0.f, 0.f};
+  collide_force<L, COLL>(f, rho, u, g, VSX_FORCE_NONE, c.p.omega, c.p.mrt_rates);
+}
+
 // ------------------------------------------------------------------ kernels
 // Bulk: one thread per cell; blockIdx.x tiles the contiguous axis, blockIdx.y
 // (and z in 3D) the others, so a wavefront touches 64 consecutive cells.
