@@ -345,3 +345,27 @@ def test_characteristic():
     assert np.allclose(r, 1 - 0.5 / cs * cs ** 2 * a)
     with pytest.raises(ValueError):
         L.boundary_characteristic(rho, u, "front")
+
+
+def test_mrt_moment_space_equilibrium():
+    """The unforced moment-space MRT of the fused kernels (cell.hpp
+    mrt_d2q9_unforced) uses the equilibrium moments of the Lallemand-Luo
+    basis in closed form; M feq(rho, u) must equal them, and f + M^-1 S (meq - M f)
+    must equal the population-space collision_mrt."""
+    rng = np.random.default_rng(11)
+    rho = 1 + 0.05 * rng.standard_normal((6, 5))
+    u = 0.08 * rng.standard_normal((2, 6, 5))
+    feq = L.get_equilibrium(rho, u)
+    ux, uy = u
+    u2 = ux * ux + uy * uy
+    meq = np.stack([rho, rho * (3 * u2 - 2), rho * (1 - 3 * u2), rho * ux, -rho * ux, rho * uy, -rho * uy,
+                    rho * (ux * ux - uy * uy), rho * ux * uy])
+    M = L.M_MRT.astype(np.float64)
+    assert np.abs(np.tensordot(M, feq, axes=([1], [0])) - meq).max() < 1e-15
+    f = feq + 1e-3 * rng.standard_normal(feq.shape)
+    omega = 1.7
+    S = L.mrt_relaxation_rates(omega)
+    m = np.tensordot(M, f, axes=([1], [0]))
+    moment_form = f + np.tensordot(L.M_MRT_INV, S[:, None, None] * (meq - m), axes=([1], [0]))
+    ref = L.collision_mrt(f, feq, L.get_mrt_collision_operator(omega))
+    assert np.abs(moment_form - ref).max() < 1e-15

@@ -250,6 +250,45 @@ __device__ __forceinline__ void mrt_d2q9(float (&f)[9], const float (&feq)[9], c
 }
 
 
+// Unforced D2Q9 MRT in moment space: f += M^-1 S (meq - M f), with the
+// equilibrium moments of this basis in closed form,
+//   meq = (rho, rho (3|u|^2 - 2), rho (1 - 3|u|^2), rho ux, -rho ux, rho uy,
+//          -rho uy, rho (ux^2 - uy^2), rho ux uy)  (= M feq(rho, u) exactly),
+// and the conserved rows (rho, jx, jy: meq - m = 0) left out.  Equal to
+// mrt_d2q9<false>(f, feq(rho, u), ...) up to float32 rounding.
+__device__ __forceinline__ void mrt_d2q9_unforced(float (&f)[9], float rho, const float (&u)[3],
+                                                  const float (&rates)[9]) {
+  const float ux = u[0], uy = u[1], u2 = ux * ux + uy * uy;
+  const float jx = rho * ux, jy = rho * uy;
+  const float meq[9] = {rho, rho * (3.f * u2 - 2.f), rho * (1.f - 3.f * u2), jx, -jx, jy, -jy,
+                        rho * (ux * ux - uy * uy), jx * uy};
+  float a[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    if (k == 0 || k == 3 || k == 5) {
+      a[k] = 0.f;
+      continue;
+    }
+    a[k] = rates[k] * (meq[k] - mrow(k, f)) * Mrt2::inv_norm[k];
+  }
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    float s = 0.f;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      if (k == 0 || k == 3 || k == 5) continue;
+      const int m = Mrt2::M[k][i];
+      if (m == 1) s += a[k];
+      if (m == -1) s -= a[k];
+      if (m == 2) s += 2.f * a[k];
+      if (m == -2) s -= 2.f * a[k];
+      if (m == 4) s += 4.f * a[k];
+      if (m == -4) s -= 4.f * a[k];
+    }
+    f[i] += s;
+  }
+}
+
 // ---------------------------------------------------------------- D3Q19 MRT in moment space
 // Raw-moment basis of lbm3d/collision/mrt.py:7-30 (rows: rho; j; |c|^2;
 // 3cx^2-|c|^2; cy^2-cz^2; cxcy, cxcz, cycz; six third-order; three fourth-order).

@@ -363,8 +363,15 @@ template <class L, int COLL>
 __device__ __forceinline__ void finish_unforced(const StepCtx &c, float (&f)[L::Q]) {
   float rho, u[3];
   macroscopic<L>(f, rho, u);
-  const float g[3] = {0.f, 0.f, 0.f};
-  collide_force<L, COLL>(f, rho, u, g, VSX_FORCE_NONE, c.p.omega, c.p.mrt_rates);
+  if constexpr (COLL == VSX_COLL_MRT && L::Q == 9) {
+    float r[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) r[k] = c.p.mrt_rates[k];
+    mrt_d2q9_unforced(f, rho, u, r);
+  } else {
+    const float g[3] = {0.f, 0.f, 0.f};
+    collide_force<L, COLL>(f, rho, u, g, VSX_FORCE_NONE, c.p.omega, c.p.mrt_rates);
+  }
 }
 
 // ------------------------------------------------------------------ kernels
