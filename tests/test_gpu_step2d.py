@@ -275,10 +275,13 @@ def test_viv_split_runs(sim):
 
 
 def test_viv_zero_iterations_is_plain_step(sim):
-    """With n_iter = 0 the IB adds nothing: the VIV run equals the plain fused run."""
+    """With n_iter = 0 the IB adds nothing: the VIV run equals the plain fused run
+    to float32 rounding (its window cells take the forced collision path,
+    every other cell the unforced one)."""
     pc = small_case(sim.VIVCase, n_iter=0)
     f0 = dev(small_case(S.VIV2D).initial_f(np.float32))
     a, _, hh = sim.VIV2D(pc).run(f0, 10)
     b = sim.LBM((pc.nx, pc.ny), pc.omega, collision=pc.collision, walls=pc.walls(), forcing=pc.forcing).run(f0, 10)
-    assert torch.equal(a, b)
+    e = assert_close(a, to_np(b), 1e-6, "viv n_iter=0 vs plain")
+    say(f"max rel diff {e:.2e}")
     assert torch.all(hh == 0)

@@ -348,15 +348,6 @@ __device__ __forceinline__ void store_cell(const StepCtx &c, typename DST::T *__
   for (int i = 0; i < L::Q; ++i) DST::st(dst + (size_t)i * c.n, cell, f[i], c.shift[i]);
 }
 
-template <class L, int COLL>
-__device__ __forceinline__ void finish(const StepCtx &c, uint32_t cell, const int (&co)[3], const int (&org)[3],
-                                       bool win, float (&f)[L::Q]) {
-  float rho, u[3], g[3];
-  macroscopic<L>(f, rho, u);
-  body_force<L>(c, cell, co, org, win, g);
-  collide_force<L, COLL>(f, rho, u, g, c.p.forcing, c.p.omega, c.p.mrt_rates);
-}
-
 // finish() of a cell that carries no body force (no g field, no uniform g, not
 // in the IB window): the forced branches are compiled out.
 template <class L, int COLL>
@@ -372,6 +363,22 @@ __device__ __forceinline__ void finish_unforced(const StepCtx &c, float (&f)[L::
     const float g[3] = {0.f, 0.f, 0.f};
     collide_force<L, COLL>(f, rho, u, g, VSX_FORCE_NONE, c.p.omega, c.p.mrt_rates);
   }
+}
+
+template <class L, int COLL>
+__device__ __forceinline__ void finish(const StepCtx &c, uint32_t cell, const int (&co)[3], const int (&org)[3],
+                                       bool win, float (&f)[L::Q]) {
+  bool unforced = !win && !c.p.g;
+#pragma unroll
+  for (int a = 0; a < L::D; ++a) unforced = unforced && c.p.g_uniform[a] == 0.f;
+  if (unforced) {
+    finish_unforced<L, COLL>(c, f);
+    return;
+  }
+  float rho, u[3], g[3];
+  macroscopic<L>(f, rho, u);
+  body_force<L>(c, cell, co, org, win, g);
+  collide_force<L, COLL>(f, rho, u, g, c.p.forcing, c.p.omega, c.p.mrt_rates);
 }
 
 // ------------------------------------------------------------------ kernels
