@@ -62,6 +62,25 @@ def test_batch_matches_oracle(sim, tile_y):
         assert np.allclose(d_, d, rtol=1e-4, atol=1e-12)
 
 
+@pytest.mark.parametrize("storage,tol", [("f16s", 3e-3), ("bf16s", 3e-2), ("e3m12s", 1.5e-3)])
+def test_batch_16bit_matches_oracle(sim, storage, tol):
+    """The batch in 16-bit shifted storage (each case keeps f - feq(1, (U0, 0)))."""
+    steps = 16
+    v0 = np.array([0.0, 1e-3])
+    pcs = [sweep_case(sim.VIVCase, *s) for s in SHAPES]
+    ocs = [sweep_case(S.VIV2D, *s) for s in SHAPES]
+    say(f"batch of {len(pcs)} {storage}")
+    b = sim.VIVBatch(pcs, storage=storage, tile_y=64, hist_cap=steps)
+    b.set_bodies(v=v0)
+    f0 = torch.zeros(b.total, dtype=torch.float32, device="cuda")
+    for k, c in enumerate(ocs):
+        b.view(f0, k).copy_(dev(c.initial_f(np.float64)))
+    got = b.run(f0, steps)
+    b.check()
+    for k, (f, d, v, a, dh, hh) in enumerate(oracle_runs(ocs, steps, v0)):
+        assert_close(b.view(got, k), f, tol, f"case {k} f ({storage})")
+
+
 def test_batch_equals_single_case_runs(sim):
     """A batch is the set of its cases: each case equals its own VIV2D run."""
     steps = 12

@@ -212,6 +212,22 @@ def test_sphere_matches_oracle(sim, variant):
         assert torch.all(hh == 0)
 
 
+@pytest.mark.parametrize("storage,tol", [("f16s", 3e-3), ("e3m12s", 1.5e-3)])
+def test_sphere_16bit_matches_oracle(sim, storage, tol):
+    """The sphere step in 16-bit shifted storage (f - feq(1, (U0, 0, 0)) kept)."""
+    oc, pc = small_sphere(S.Sphere3D), small_sphere(sim.SphereCase)
+    steps = 12
+    say(f"sphere {storage}")
+    solver = sim.Sphere3D(pc, storage=storage)
+    f0 = oc.initial_f(np.float64)
+    got, hh = solver.run(dev(f0), steps)
+    W, I = oc.stencil(np.float32)
+    f = f0.astype(np.float64)
+    for _ in range(steps):
+        f, h = S.sphere_step(oc, f, W.astype(np.float64), I)
+    assert_close(got, f, tol, f"sphere f ({storage})")
+
+
 def test_sphere_transpose_is_exact(sim):
     """The transposed stencil lists every in-range slot once, per cell in ascending order."""
     pc = small_sphere(sim.SphereCase)
