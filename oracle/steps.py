@@ -112,7 +112,7 @@ def cavity_walls(U0, order=("left", "right", "bottom", "top")):
 @dataclass
 class VIV2D:
     """Configuration of the 2D vortex-induced-vibration IB-LBM case
-    (examples/2d/vortex_induced_vibration.py:31-74, with BASELINE's sizes).
+    (examples/2d/vortex_induced_vibration.py:31-74, defaults: the 1024 x 1024 benchmark case).
 
     ``dof``: which rigid-body DOFs move (x, y); inactive DOFs stay at zero.
     """
@@ -285,8 +285,8 @@ def viv_run(case, f, n_steps, dtype=None, d=None, v=None, a=None):
 
 @dataclass
 class Sphere3D:
-    """Flow past a fixed sphere (examples/3d/flow_past_sphere.py:29-124) at BASELINE's
-    sizes; markers on a Fibonacci lattice, uniform ds = pi D^2 / N."""
+    """Flow past a fixed sphere (examples/3d/flow_past_sphere.py:29-124), defaults: the
+    256 x 128 x 128 benchmark case; markers on a Fibonacci lattice, uniform ds = pi D^2 / N."""
     nx: int = 256
     ny: int = 128
     nz: int = 128

@@ -5,8 +5,8 @@ coefficient history, the transverse displacement history and the final
 velocity field, for FP32 and the 16-bit shifted storages (FP32 arithmetic: FP16, BF16 and
 the E3M12 code), which keep f_i - feq_i(1, (U0, 0)); "f16s_rest" keeps
 f_i - w_i instead.
-The measured values are written to gpurun_out/accuracy.json when that
-folder exists."""
+The measured values are also written as JSON (accuracy.json) into the run's
+output folder when it exists."""
 
 import json
 import os
