@@ -113,4 +113,22 @@ struct Store<VSX_STORE_E3M12S> {
 
 inline size_t store_bytes(int s) { return s == VSX_STORE_F32 ? 4 : 2; }
 
+// ------------------------------------------------------------------ hand-off between workgroups
+// Workgroups of one launch hand each other small buffers (the IB window
+// velocity and force window).  Each access below is coherent at agent scope
+// by itself (relaxed agent-scope atomics: the L2 of the XCD is bypassed), so
+// a hand-off is: the writer's threads store with st_agent, wait until their
+// stores have completed (wait_mem), meet at a barrier, and one thread sets a
+// flag; the reader polls the flag and loads with ld_agent.  No whole-L2
+// writeback or invalidation (what an agent-scope fence costs on gfx950,
+// whose XCDs have separate L2s) is needed.
+__device__ __forceinline__ void st_agent(float *p, float v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float ld_agent(const float *p) {
+  return __hip_atomic_load(const_cast<float *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// wait until every memory operation of this thread has completed
+__device__ __forceinline__ void wait_mem() { __builtin_amdgcn_s_waitcnt(0); }
+
 }  // namespace vsx

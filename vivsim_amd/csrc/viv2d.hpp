@@ -526,8 +526,8 @@ __device__ __forceinline__ void window_store(float2 *grid, int base, const float
 }
 
 // Wait (thread 0, bounded) until the n_wtiles window tiles of step gstep have
-// published the window velocity in ib.g_win, reset their counter for step
-// gstep + 2, then acquire at agent scope.
+// published the window velocity in ib.g_win (read it with ld_agent), and
+// reset their counter for step gstep + 2.
 __device__ __forceinline__ void wait_window_u(const VsxIB2D &ib, int gstep, int n_wtiles) {
   if (threadIdx.x == 0) {
     int32_t *cnt_u = &ib.win_origin[4 + (gstep & 1)];
@@ -535,7 +535,6 @@ __device__ __forceinline__ void wait_window_u(const VsxIB2D &ib, int gstep, int 
     __hip_atomic_store(cnt_u, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
 // window values per thread held in registers between the wait for the window
@@ -692,7 +691,7 @@ __device__ __forceinline__ void ib_block(const StepCtx &c, const VsxIB2D &ib, co
 #pragma unroll
         for (int j = 0; j < kUPrefetch; ++j) {
           const int w = tid + j * nt;   // interleaved grid word: cell w / 2, component w % 2
-          upf[j] = w < 2 * SS ? ib.g_win[(w & 1) * SS + (w >> 1)] : 0.f;
+          upf[j] = w < 2 * SS ? ld_agent(&ib.g_win[(w & 1) * SS + (w >> 1)]) : 0.f;
         }
       }
     }
@@ -712,7 +711,7 @@ __device__ __forceinline__ void ib_block(const StepCtx &c, const VsxIB2D &ib, co
       }
     } else {
       if (!BINS) wait_window_u(ib, gstep, n_wtiles);
-      for (int k = tid; k < SS; k += nt) grid[k] = make_float2(ib.g_win[k], ib.g_win[SS + k]);
+      for (int k = tid; k < SS; k += nt) grid[k] = make_float2(ld_agent(&ib.g_win[k]), ld_agent(&ib.g_win[SS + k]));
     }
   } else {
     for (int base = 0; base < SS; base += UW * nt) {
@@ -815,8 +814,8 @@ __device__ __forceinline__ void ib_block(const StepCtx &c, const VsxIB2D &ib, co
   // force window -> global
   for (int k = tid; k < SS; k += nt) {
     const float2 g = grid[k];
-    ib.g_win[k] = g.x;
-    ib.g_win[SS + k] = g.y;
+    st_agent(&ib.g_win[k], g.x);
+    st_agent(&ib.g_win[SS + k], g.y);
   }
 
   // body force h = sum(-F) (FP64)
@@ -864,23 +863,20 @@ __device__ __forceinline__ void ib_block(const StepCtx &c, const VsxIB2D &ib, co
   }
 }
 
-// Publish "the force window of this step is written" by storing `value` (the
-// global step index + 1) to the ready flag with an agent-scope release, so the
-// window tiles on other XCDs see the stores.
+// Publish "the force window of this step is written" (its st_agent stores
+// have completed in every thread) by storing `value` (the global step index
+// + 1) to the ready flag.
 __device__ __forceinline__ void publish_ready(const VsxIB2D &ib, int value) {
+  wait_mem();
   __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    __hip_atomic_store(&ib.win_origin[2], value, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  if (threadIdx.x == 0) __hip_atomic_store(&ib.win_origin[2], value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Wait of a window tile until the ready flag reaches `target` (bounded, see
-// spin_until); then every thread of the tile acquires at agent scope.
+// spin_until); the force window is then read with ld_agent.
 __device__ __forceinline__ void wait_ready(const VsxIB2D &ib, int target) {
   if (threadIdx.x == 0) spin_until(ib, &ib.win_origin[2], target);
   __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
 // Window tile: store the post-streaming velocity of a cell that lies in the
@@ -892,21 +888,21 @@ __device__ __forceinline__ void store_window_u(const VsxIB2D &ib, bool valid, co
   if (valid && r0 >= 0 && r0 < S && r1 >= 0 && r1 < S) {
     float rho, u[3];
     macroscopic<D2Q9>(f, rho, u);
-    ib.g_win[r0 * S + r1] = u[0];
-    ib.g_win[S * S + r0 * S + r1] = u[1];
+    st_agent(&ib.g_win[r0 * S + r1], u[0]);
+    st_agent(&ib.g_win[S * S + r0 * S + r1], u[1]);
   }
 }
 
-// ... then count the tile in ib.win_origin[4 + (gstep & 1)] with an
-// agent-scope release; the last of the n_wtiles window tiles also raises
-// ib.win_origin[6] to gstep + 1 (the bulk tiles start their own traffic after
-// it, so the window pulls are served first).
+// ... then, once every thread's stores have completed, count the tile in
+// ib.win_origin[4 + (gstep & 1)]; the last of the n_wtiles window tiles also
+// raises ib.win_origin[6] to gstep + 1 (the bulk tiles start their own
+// traffic after it, so the window pulls are served first).
 __device__ __forceinline__ void count_window_tile(const VsxIB2D &ib, int gstep, int n_wtiles) {
+  wait_mem();
   __syncthreads();
   if (threadIdx.x == 0) {
-    __threadfence();
     const int before =
-        __hip_atomic_fetch_add(&ib.win_origin[4 + (gstep & 1)], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(&ib.win_origin[4 + (gstep & 1)], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (before == n_wtiles - 1)
       __hip_atomic_store(&ib.win_origin[6], gstep + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
