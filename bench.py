@@ -33,7 +33,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_viv_fused launch on the 1024^2 FP32 workload, from rocprofv3 FETCH_SIZE / WRITE_SIZE
 # passes scaled by a 256 MiB copy (profiles/k_viv_fused_viv2d_1024x1024_f32_pmc.md); a profiler run
 # cannot be part of the timed bench itself.
-TRAFFIC_1024_F32 = 79691392
+TRAFFIC_1024_F32 = 79692224
 
 
 def kernel_times(lib, n):

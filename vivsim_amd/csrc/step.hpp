@@ -271,7 +271,12 @@ __device__ __forceinline__ void body_force(const StepCtx &c, uint32_t cell, cons
       for (int a = 1; a < L::D; ++a) k = k * c.win.size[a] + r[a];
       const int wn = c.win.size[0] * c.win.size[1] * (L::D == 3 ? c.win.size[2] : 1);
 #pragma unroll
-      for (int a = 0; a < L::D; ++a) g[a] += ld_agent(&c.win.g[a * wn + k]);
+      for (int a = 0; a < L::D; ++a) {
+        // 2D: the force window may come from the IB workgroup of this launch;
+        // 3D: it was written by an earlier launch
+        if constexpr (L::D == 2) g[a] += ld_agent(&c.win.g[a * wn + k]);
+        else g[a] += c.win.g[a * wn + k];
+      }
     }
   }
 }
