@@ -369,3 +369,48 @@ def test_mrt_moment_space_equilibrium():
     moment_form = f + np.tensordot(L.M_MRT_INV, S[:, None, None] * (meq - m), axes=([1], [0]))
     ref = L.collision_mrt(f, feq, L.get_mrt_collision_operator(omega))
     assert np.abs(moment_form - ref).max() < 1e-15
+
+
+def _cbc_profiles(shape, na, dt=np.float64):
+    """Quadratic profiles along the normal axis na (coefficients varying across
+    the wall) and their exact derivatives d/dx_na, as functions of the index grid."""
+    idx = np.meshgrid(*[np.arange(n, dtype=dt) for n in shape], indexing="ij")
+    x = idx[na]
+    t = sum(idx[a] for a in range(len(shape)) if a != na)
+    a_r, b_r = 0.01 + 0.001 * t, 0.002
+    a_u, b_u = 0.004 + 0.0005 * t, -0.001
+    rho = 1 + a_r * x + b_r * x * x
+    un = 0.03 + 0.002 * t + a_u * x + b_u * x * x
+    return x, t, rho, un, a_r + 2 * b_r * x, a_u + 2 * b_u * x
+
+
+@pytest.mark.parametrize("loc", LOCS)
+def test_characteristic_invariants(loc):
+    """Pins boundary/cbc.py:14-53 by the characteristic form of its update with
+    exact derivatives.  With J+- = u_n +- cs ln(rho) linearised as
+    dJ+- = du_n +- (cs / rho) drho and lambda = u_n - s cs (s = inward sign):
+      du_n - (cs / rho) drho = 0 and du_n + (cs / rho) drho = -lambda (u_n' - s cs / rho rho'),
+    where ' is the exact derivative along the axis of the quadratic profiles
+    (the 3-point one-sided difference of the reference is exact on them).  At
+    s = -1 this is the outgoing right-running characteristic advected with
+    u + cs while the incoming one is held; at s = +1 the reference applies the
+    left-running advection term to the same combination.  The tangential
+    velocity is copied from the wall line."""
+    s = LT.WALLS2[loc]
+    na, sg = s["axis"], s["sign"]
+    shape = (7, 6)
+    x, t, rho, un, drho, dun = _cbc_profiles(shape, na)
+    ut = 0.01 + 0.003 * t + 0.005 * x
+    u = np.zeros((2,) + shape)
+    u[na], u[1 - na] = un, ut
+    r_next, u_next = L.boundary_characteristic(rho, u, loc)
+    w = 0 if sg > 0 else -1
+    take = lambda a: np.take(a, w, axis=na)
+    r1, n1, dr, dn = take(rho), take(un), take(drho), take(dun)
+    cs = 1 / np.sqrt(3)
+    lam = n1 - sg * cs
+    d_rho, d_un = r_next - r1, u_next[na] - n1
+    assert np.abs(d_un - cs / r1 * d_rho).max() < 1e-13
+    assert np.abs((d_un + cs / r1 * d_rho) + lam * (dn - sg * cs / r1 * dr)).max() < 1e-13
+    assert np.abs(d_un).min() > 1e-5                       # the case moves the state
+    assert np.array_equal(u_next[1 - na], take(ut))

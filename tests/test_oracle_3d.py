@@ -235,3 +235,76 @@ def test_sphere_drag_points_downstream():
     for _ in range(3):
         f, h = S.sphere_step(c, f)
     assert h[0] > 0 and abs(h[1]) < 1e-2 * h[0] and abs(h[2]) < 1e-2 * h[0]
+
+
+FACES3 = ("left", "right", "bottom", "top", "back", "front")
+
+
+@pytest.mark.parametrize("loc", FACES3)
+def test_characteristic_invariants_3d(loc):
+    """Pins lbm3d/boundary/cbc.py:16-51 the way the 2D test pins cbc.py: with
+    lambda = u_n - s cs, du_n - (cs / rho) drho = 0 and
+    du_n + (cs / rho) drho = -lambda (u_n' - s cs / rho rho') with the exact
+    derivatives of quadratic profiles; both tangential components are copied
+    from the face."""
+    from tests.test_oracle_2d import _cbc_profiles
+    s = LT.WALLS3[loc]
+    na, sg, ta = s["axis"], s["sign"], s["tan_axes"]
+    shape = (6, 5, 4)
+    x, t, rho, un, drho, dun = _cbc_profiles(shape, na)
+    u = np.zeros((3,) + shape)
+    u[na] = un
+    u[ta[0]] = 0.01 + 0.003 * t + 0.005 * x
+    u[ta[1]] = -0.02 + 0.001 * t - 0.004 * x
+    r_next, u_next = L.boundary_characteristic(rho, u, loc)
+    w = 0 if sg > 0 else -1
+    take = lambda a: np.take(a, w, axis=na)
+    r1, n1, dr, dn = take(rho), take(un), take(drho), take(dun)
+    cs = np.sqrt(1 / 3)
+    d_rho, d_un = r_next - r1, u_next[na] - n1
+    assert np.abs(d_un - cs / r1 * d_rho).max() < 1e-13
+    assert np.abs((d_un + cs / r1 * d_rho) + (n1 - sg * cs) * (dn - sg * cs / r1 * dr)).max() < 1e-13
+    assert np.abs(d_un).min() > 1e-5
+    for a in ta:
+        assert np.array_equal(u_next[a], take(u[a]))
+
+
+@pytest.mark.parametrize("loc", FACES3)
+@pytest.mark.parametrize("kind", ["nee", "nebb", "equilibrium"])
+def test_pressure_wrappers_3d_mass_balance(loc, kind):
+    """Pins the pressure wrappers (lbm3d/boundary/_helpers.py:66-77, 105-116)
+    by the mass balance they impose, computed here from the lattice moments
+    alone.  For a uniform equilibrium feq(rho0, u0) the face sums satisfy
+    sum_zero + 2 sum_out = rho0 (1 - u0 . n_in) (n_in the inward normal), so a
+    prescribed face density rho1 fixes the inward normal velocity
+    1 - (rho0 / rho1)(1 - u0 . n_in).  After each pressure wrapper the face has
+    density rho1 and that normal velocity; EQ and NEE also carry the
+    neighbour's tangential velocity (checked on a state whose tangential
+    velocity changes along the normal); with rho1 = rho0 a uniform
+    equilibrium is unchanged."""
+    s = LT.WALLS3[loc]
+    na, sg, ta = s["axis"], s["sign"], s["tan_axes"]
+    shape = (5, 4, 6)
+    rho0, u0 = 1.01, np.array([0.03, -0.02, 0.015])
+    uf = np.broadcast_to(u0[:, None, None, None], (3,) + shape).copy()
+    f = L.get_equilibrium(np.full(shape, rho0), uf)
+    # the same with tangential velocities that change along the normal (face != neighbour)
+    xn = np.meshgrid(*[np.arange(n, dtype=float) for n in shape], indexing="ij")[na]
+    ug = uf.copy()
+    for k, a in enumerate(ta):
+        ug[a] += (0.004 - 0.007 * k) * xn
+    g = L.get_equilibrium(np.full(shape, rho0), ug)
+    fn = getattr(L, f"boundary_pressure_{kind}")
+    un_in = sg * u0[na]
+    w, nb = (0, 1) if sg > 0 else (-1, -2)
+    for state, uu in ((f, uf), (g, ug)):
+        for rho1 in (rho0, 0.997):
+            out = fn(state, loc, rho_wall=rho1)
+            r, u = L.get_macroscopic(np.take(out, w, axis=1 + na))
+            assert np.abs(r - rho1).max() < 1e-13
+            assert np.abs(sg * u[na] - (1 - rho0 / rho1 * (1 - un_in))).max() < 1e-13
+            if kind != "nebb":
+                for a in ta:
+                    assert np.abs(u[a] - np.take(uu[a], nb, axis=na)).max() < 1e-13
+            if rho1 == rho0 and state is f:
+                assert np.abs(out - f).max() < 1e-15

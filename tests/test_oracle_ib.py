@@ -192,3 +192,30 @@ def test_kinematics():
     ht = np.stack([-np.sin(th), np.cos(th)], 1)           # unit tangential forces on a unit circle
     assert np.isclose(DYN.get_torque_to_obj(x, y, 5.0, 6.0, np.zeros(3), ht), 8.0)
     assert np.allclose(DYN.get_force_to_obj(ht), 0)
+
+
+def test_markers_velocity_3dof_is_time_derivative_of_coords():
+    """Pins dyn.py:103-120 against the kinematics it must satisfy: along the
+    rigid motion d(t) = d0 + v t, the marker coordinates of
+    get_markers_coords_3dof (dyn.py:84-100) move with d/dt X = v + omega x r.
+    A central difference in float64 checks sign, scale and the v0 / v1 terms;
+    the closed-form part checks |v_m - v_c| = |omega| r, counter-clockwise for
+    omega > 0."""
+    th = 2 * np.pi * np.arange(12) / 12 + 0.1
+    x0, y0 = 3.0 + 1.7 * np.cos(th), -2.0 + 1.7 * np.sin(th)
+    d0 = np.array([0.4, -0.3, 0.25])
+    v = np.array([0.07, -0.05, 0.9])
+    h = 1e-6
+    xp, yp = DYN.get_markers_coords_3dof(x0, y0, 3.0, -2.0, d0 + h * v)
+    xm, ym = DYN.get_markers_coords_3dof(x0, y0, 3.0, -2.0, d0 - h * v)
+    fd = np.stack([(xp - xm) / (2 * h), (yp - ym) / (2 * h)], axis=-1)
+    xs, ys = DYN.get_markers_coords_3dof(x0, y0, 3.0, -2.0, d0)
+    vm = DYN.get_markers_velocity_3dof(xs, ys, 3.0, -2.0, d0, v)
+    assert np.abs(vm - fd).max() < 1e-8
+    rel = vm - v[:2]
+    rx, ry = xs - (3.0 + d0[0]), ys - (-2.0 + d0[1])
+    assert np.allclose(np.hypot(rel[:, 0], rel[:, 1]), abs(v[2]) * np.hypot(rx, ry), rtol=1e-12)
+    assert np.all(rx * rel[:, 1] - ry * rel[:, 0] > 0)      # omega > 0: counter-clockwise
+    # pure translation: every marker moves with (v0, v1)
+    vt = DYN.get_markers_velocity_3dof(xs, ys, 3.0, -2.0, d0, np.array([0.07, -0.05, 0.0]))
+    assert np.allclose(vt, [0.07, -0.05], rtol=0, atol=0)
