@@ -110,8 +110,9 @@ typedef struct VsxIB2D {
   int32_t *win_origin; /* device int32 [8]: [0..1] window origin x, y of the last step;
                           [2] ready flag (= step0 + t + 1 once step t's force window is
                           written); [3] error flag (1 if a bounded wait ran out); [4..5]
-                          window-tile counters of the fused step; [6] = step0 + t + 1 once
-                          all window tiles of step t have published; [7] reserved           */
+                          window-tile counters of the batched step; [6] = step0 + t + 1 once
+                          the IB workgroup of step t has pulled its window (the bulk tiles of
+                          the fused step wait for it); [7] reserved                          */
   int32_t step0;       /* global index of the first step of this run (0 after a reset of the
                           body state, which must also zero win_origin)                       */
   int32_t fused;       /* 1: one launch per step, the IB workgroup overlapped with the bulk

@@ -124,3 +124,38 @@ def test_api_surface_matches_reference_names():
     for n in ["newmark", "newmark_2dof", "newmark_3dof", "get_markers_coords_2dof", "get_markers_coords_3dof",
               "get_markers_velocity_3dof", "get_force_to_obj", "get_torque_to_obj"]:
         assert callable(getattr(V.dyn, n))
+
+
+def test_signatures_match_reference():
+    """Every public function of the drop-in modules has the reference's
+    parameter names, order, kinds and defaults.  The reference signatures are
+    a committed fixture (tests/golden/reference_signatures.json), read from
+    the reference source text by tests/golden/make_signatures.py."""
+    import inspect
+    import json
+
+    import vivsim_amd as V
+    ref = json.load(open(os.path.join(ROOT, "tests", "golden", "reference_signatures.json")))
+    kinds = {"positional": (inspect.Parameter.POSITIONAL_OR_KEYWORD, inspect.Parameter.POSITIONAL_ONLY),
+             "var_positional": (inspect.Parameter.VAR_POSITIONAL,),
+             "keyword_only": (inspect.Parameter.KEYWORD_ONLY,),
+             "var_keyword": (inspect.Parameter.VAR_KEYWORD,)}
+    checked = 0
+    for modname, funcs in ref.items():
+        mod = getattr(V, modname)
+        for fname, params in funcs.items():
+            ours = inspect.signature(getattr(mod, fname))
+            got = list(ours.parameters.values())
+            assert [p.name for p in got] == [p["name"] for p in params], (modname, fname, str(ours))
+            for p, r in zip(got, params):
+                assert p.kind in kinds[r["kind"]], (modname, fname, p.name, p.kind)
+                d = r["default"]
+                if d is None:
+                    assert p.default is inspect.Parameter.empty, (modname, fname, p.name)
+                elif "name" in d:
+                    assert getattr(p.default, "__name__", None) == d["name"], (modname, fname, p.name, p.default)
+                else:
+                    assert p.default == d["value"] and type(p.default) is type(d["value"]), \
+                        (modname, fname, p.name, p.default, d["value"])
+            checked += 1
+    assert checked == sum(len(v) for v in ref.values()) >= 90

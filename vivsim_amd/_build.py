@@ -2,7 +2,8 @@
 
 Each ``csrc/*.hip`` file is compiled to an object with ``hipcc`` in parallel,
 then linked into ``vivsim_amd/libvivsim_amd.so`` (in-tree, so it travels with
-the repository to the GPU host).  ``python -m vivsim_amd._build`` runs it.
+the repository to the GPU host).  ``python -m vivsim_amd._build`` runs it
+(``--changed``: recompile only the sources whose includes changed).
 """
 
 import concurrent.futures as cf
@@ -28,8 +29,24 @@ def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
 
 
-def _compile(src):
+def _deps(path, seen=None):
+    """The file and every file it includes with #include "..." (recursively)."""
+    seen = set() if seen is None else seen
+    path = os.path.normpath(path)
+    if path in seen or not os.path.exists(path):
+        return seen
+    seen.add(path)
+    for line in open(path):
+        line = line.strip()
+        if line.startswith("#include \""):
+            _deps(os.path.join(os.path.dirname(path), line.split('"')[1]), seen)
+    return seen
+
+
+def _compile(src, force=True):
     obj = os.path.join(OBJ, os.path.basename(src)[:-4] + ".o")
+    if not force and os.path.exists(obj) and os.path.getmtime(obj) > max(os.path.getmtime(d) for d in _deps(src)):
+        return obj
     cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -38,11 +55,13 @@ def _compile(src):
     return obj
 
 
-def build(verbose=True):
+def build(verbose=True, force=True):
+    """Compile every source (force=False: only those whose object is older than
+    the source or a file it includes) and link the library."""
     os.makedirs(OBJ, exist_ok=True)
     srcs = sources()
     with cf.ThreadPoolExecutor(max_workers=min(len(srcs), 6)) as ex:
-        objs = list(ex.map(_compile, srcs))
+        objs = list(ex.map(lambda x: _compile(x, force), srcs))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -55,5 +74,5 @@ def build(verbose=True):
 
 
 if __name__ == "__main__":
-    build()
+    build(force="--changed" not in sys.argv)
     sys.exit(0)

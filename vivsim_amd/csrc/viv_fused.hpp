@@ -1,0 +1,337 @@
+// The fused 2D VIV step: one launch per reference time step.
+//
+// Blocks: block 0 is the IB workgroup (ib_atomic); blocks 1 .. n_win are the
+// tiles that intersect the force window; the remaining blocks are bulk
+// tiles.  Every block derives the step's window origin from the body state
+// before the step (slot gstep & 1), so the step needs no host sync.
+//   IB workgroup  pulls the window's post-streaming velocity itself, raises
+//                 ib.win_origin[6] (the bulk tiles start streaming after it,
+//                 so the window pulls are not queued behind the whole grid's
+//                 traffic), runs the multi-direct forcing, writes the force
+//                 window with st_agent and raises the ready flag
+//                 ib.win_origin[2]
+//   window tiles  pull their cells, wait (bounded) for the ready flag, read the
+//                 force window with ld_agent, collide
+//   bulk tiles    wait (bounded) for ib.win_origin[6], then stream and collide
+// Modes (the post-collision state is what the work buffers hold):
+//   kModeFull   src = post-collision state of the previous step (pull + walls)
+//   kModeFirst  src = the caller's post-streaming FP32 field, read in place (the
+//               reference state a run starts from: no pull, walls already applied)
+//   kModeLast   the new post-collision populations are pushed to their
+//               streaming destinations in an FP32 field (the caller's output);
+//               the walls are applied afterwards by k_wall_line, wall by wall
+#pragma once
+#include "viv2d.hpp"
+
+namespace vsx {
+
+constexpr int kModeFull = 0, kModeFirst = 1, kModeLast = 2;
+
+// Push-order store of a collided cell: f_i goes to x + c_i (periodic), which is
+// the reference's streamed field before the walls.  Destination cells are
+// (row offset of x + cx) + (y + cy): six 32-bit values for the nine stores.
+__device__ __forceinline__ void store_push2(const StepCtx &c, float *__restrict__ dst, const int (&co)[3],
+                                            const float (&f)[9]) {
+  const uint32_t ny = (uint32_t)c.dims[1];
+  const uint32_t xr[3] = {(uint32_t)wrap(co[0] - 1, c.dims[0]) * ny, (uint32_t)co[0] * ny,
+                          (uint32_t)wrap(co[0] + 1, c.dims[0]) * ny};
+  const uint32_t yr[3] = {(uint32_t)wrap(co[1] - 1, c.dims[1]), (uint32_t)co[1], (uint32_t)wrap(co[1] + 1, c.dims[1])};
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    float *p = dst + (size_t)i * c.n;   // uniform base + 32-bit cell offset
+    p[xr[D2Q9::cx[i] + 1] + yr[D2Q9::cy[i] + 1]] = f[i];
+  }
+}
+
+// Post-streaming, post-walls populations of a tile cell (kModeFirst: in place).
+template <class SRC, int MODE>
+__device__ __forceinline__ void tile_cell_state(const StepCtx &c, const typename SRC::T *__restrict__ src,
+                                                const int (&co)[3], float (&f)[9]) {
+  if constexpr (MODE == kModeFirst) {
+    local<D2Q9, SRC>(c, src, cell_of<D2Q9>(c, co[0], co[1], 0), f);
+  } else {
+    const int cls = c.p.n_walls > 0 ? wall_class<D2Q9>(c, co) : -1;
+    if (cls >= 0) single_wall_state<D2Q9, SRC>(c, src, co, cls, f);
+    else pull<D2Q9, SRC>(c, src, co[0], co[1], 0, f);
+  }
+}
+
+template <class DST, int MODE>
+__device__ __forceinline__ void tile_store(const StepCtx &c, typename DST::T *__restrict__ dst, const int (&co)[3],
+                                           const float (&f)[9]) {
+  if constexpr (MODE == kModeLast) store_push2(c, dst, co, f);
+  else store_cell<D2Q9, DST>(c, dst, cell_of<D2Q9>(c, co[0], co[1], 0), f);
+}
+
+// Two bulk cells per thread (16-bit storages, kModeFull): both cells'
+// populations are loaded before either is computed, so a wave keeps as many
+// bytes in flight as with one FP32 cell per thread.  a and b are interior
+// cells (no walls) outside the IB window, in a step without a g field or
+// uniform g.
+template <class T>
+struct Pack2;
+template <>
+struct Pack2<_Float16> {
+  typedef _Float16 type __attribute__((ext_vector_type(2)));
+};
+template <>
+struct Pack2<uint16_t> {
+  typedef uint16_t type __attribute__((ext_vector_type(2)));
+};
+
+template <class SRC, class DST, int COLL>
+__device__ __forceinline__ void bulk_pair(const StepCtx &c, const typename SRC::T *__restrict__ src,
+                                          typename DST::T *__restrict__ dst, const int (&a)[3], const int (&b)[3]) {
+  // the two cells' values of population i share one VGPR (low / high half)
+  typename Pack2<typename SRC::T>::type r[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    const typename SRC::T *p = src + (size_t)i * c.n;
+    r[i] = {p[cell_of<D2Q9>(c, wrap(a[0] - D2Q9::cx[i], c.dims[0]), wrap(a[1] - D2Q9::cy[i], c.dims[1]), 0)],
+            p[cell_of<D2Q9>(c, wrap(b[0] - D2Q9::cx[i], c.dims[0]), wrap(b[1] - D2Q9::cy[i], c.dims[1]), 0)]};
+  }
+  float f[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) f[i] = SRC::cvt(r[i].x, c.shift[i]);
+  const uint32_t ca = cell_of<D2Q9>(c, a[0], a[1], 0);
+  finish_unforced<D2Q9, COLL>(c, f);
+  store_cell<D2Q9, DST>(c, dst, ca, f);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) f[i] = SRC::cvt(r[i].y, c.shift[i]);
+  const uint32_t cb = cell_of<D2Q9>(c, b[0], b[1], 0);
+  finish_unforced<D2Q9, COLL>(c, f);
+  store_cell<D2Q9, DST>(c, dst, cb, f);
+}
+
+// One bulk cell (any storage, walls included).
+template <class SRC, class DST, int COLL, int MODE>
+__device__ __forceinline__ void bulk_cell(const StepCtx &c, const typename SRC::T *__restrict__ src,
+                                          typename DST::T *__restrict__ dst, const int (&co)[3], const int (&org)[3]) {
+  float f[9];
+  tile_cell_state<SRC, MODE>(c, src, co, f);
+  finish<D2Q9, COLL>(c, cell_of<D2Q9>(c, co[0], co[1], 0), co, org, false, f);
+  tile_store<DST, MODE>(c, dst, co, f);
+}
+
+// With PAIR (kModeFull only): block 0 the IB workgroup, blocks 1..n_win the
+// window tiles, then one block per two bulk tiles; the host launches
+// 1 + ceil((tiles + max window tiles) / 2) blocks and the surplus exits.
+template <class SRC, class DST, int COLL, bool PAIR, int MODE>
+__global__ __attribute__((amdgpu_flat_work_group_size(kTileCells, kTileCells), amdgpu_waves_per_eu(8, 8))) void
+k_viv_fused(StepCtx c_arg, VsxIB2D ib, const typename SRC::T *__restrict__ src, typename DST::T *__restrict__ dst,
+            int step, Tiles T) {
+  static_assert(!PAIR || MODE == kModeFull, "paired bulk blocks only in full steps");
+  const StepCtx &c = kernel_ctx();
+  extern __shared__ float lds[];
+  const int gstep = ib.step0 + step, slot = gstep & 1;
+  const double *body_in = ib.body + 6 * slot;
+  __shared__ int s_org[2];
+  if (threadIdx.x == 0) {
+    int o[4];
+    window_origin(c, ib, body_in, o);
+    s_org[0] = o[2];
+    s_org[1] = o[3];
+  }
+  __syncthreads();
+  const int org[3] = {s_org[0], s_org[1], 0};
+  int wx0, wx1, wy0, wy1;
+  window_tiles(T, org[0], org[1], ib.win, wx0, wx1, wy0, wy1);
+  if (blockIdx.x == 0) {
+    ib_atomic<SRC, MODE != kModeFirst, 2, 1>(c, ib, src, step, gstep, &ib.win_origin[6], body_in,
+                                             ib.body + 6 * (slot ^ 1), lds);
+    publish_ready(ib, gstep + 1);
+    return;
+  }
+  const int n_win = (wx1 - wx0 + 1) * (wy1 - wy0 + 1);
+  if constexpr (PAIR) {
+    if ((int)blockIdx.x - 1 >= n_win) {
+      const int q = n_win + 2 * ((int)blockIdx.x - 1 - n_win), n_tiles = T.ntx * T.nty;
+      if (q >= n_tiles) return;
+      int ta[2], tb[2];
+      bool w_;
+      tile_of_block(q, T, wx0, wx1, wy0, wy1, ta[0], ta[1], w_);
+      const bool has_b = q + 1 < n_tiles;
+      if (has_b) tile_of_block(q + 1, T, wx0, wx1, wy0, wy1, tb[0], tb[1], w_);
+      const int a[3] = {ta[0] * T.TX + (int)threadIdx.x / T.TY, ta[1] * T.TY + (int)threadIdx.x % T.TY, 0};
+      const int b[3] = {has_b ? tb[0] * T.TX + (int)threadIdx.x / T.TY : c.dims[0],
+                        has_b ? tb[1] * T.TY + (int)threadIdx.x % T.TY : c.dims[1], 0};
+      if (threadIdx.x == 0) spin_until(ib, &ib.win_origin[6], gstep + 1);
+      __syncthreads();
+      const bool va = a[0] < c.dims[0] && a[1] < c.dims[1], vb = b[0] < c.dims[0] && b[1] < c.dims[1];
+      const bool wa = va && c.p.n_walls > 0 && wall_class<D2Q9>(c, a) >= 0;
+      const bool wb = vb && c.p.n_walls > 0 && wall_class<D2Q9>(c, b) >= 0;
+      const bool unforced = !c.p.g && c.p.g_uniform[0] == 0.f && c.p.g_uniform[1] == 0.f;
+      if (unforced && va && vb && !wa && !wb) {
+        bulk_pair<SRC, DST, COLL>(c, src, dst, a, b);
+      } else {
+        if (va) bulk_cell<SRC, DST, COLL, MODE>(c, src, dst, a, org);
+        if (vb) bulk_cell<SRC, DST, COLL, MODE>(c, src, dst, b, org);
+      }
+      return;
+    }
+  }
+  int tx, ty;
+  bool in_win;
+  tile_of_block(blockIdx.x - 1, T, wx0, wx1, wy0, wy1, tx, ty, in_win);
+  const int co[3] = {tx * T.TX + (int)threadIdx.x / T.TY, ty * T.TY + (int)threadIdx.x % T.TY, 0};
+  const bool valid = co[0] < c.dims[0] && co[1] < c.dims[1];
+  float f[9];
+  if (in_win) {
+    // window tile: post-streaming state -> wait for the IB workgroup's force window -> collide
+    if (valid) tile_cell_state<SRC, MODE>(c, src, co, f);
+    wait_ready(ib, gstep + 1);
+    if (!valid) return;
+    finish<D2Q9, COLL>(c, cell_of<D2Q9>(c, co[0], co[1], 0), co, org, true, f);
+    tile_store<DST, MODE>(c, dst, co, f);
+    return;
+  }
+  // bulk tile: wait (bounded) until the IB workgroup has pulled the window
+  if (threadIdx.x == 0) spin_until(ib, &ib.win_origin[6], gstep + 1);
+  __syncthreads();
+  if (!valid) return;
+  if constexpr (MODE != kModeFirst) {
+    const int cls = c.p.n_walls > 0 ? wall_class<D2Q9>(c, co) : -1;
+    if (cls >= 0) {
+      // wall cell: own path, keeping the wall code's registers out of the interior path
+      single_wall_state<D2Q9, SRC>(c, src, co, cls, f);
+      finish<D2Q9, COLL>(c, cell_of<D2Q9>(c, co[0], co[1], 0), co, org, false, f);
+      tile_store<DST, MODE>(c, dst, co, f);
+      return;
+    }
+    pull<D2Q9, SRC>(c, src, co[0], co[1], 0, f);
+  } else {
+    local<D2Q9, SRC>(c, src, cell_of<D2Q9>(c, co[0], co[1], 0), f);
+  }
+  finish<D2Q9, COLL>(c, cell_of<D2Q9>(c, co[0], co[1], 0), co, org, false, f);
+  tile_store<DST, MODE>(c, dst, co, f);
+}
+
+// One wall applied in place to a post-streaming FP32 field (the reference's
+// boundary_* call on the whole array): thread j takes wall node j, reads the
+// wall cell and, where the scheme needs it, the inward neighbour (both already
+// carry the walls applied before this one), and rewrites the wall cell.  The
+// wall must not need the pre-streaming populations.
+template <class L>
+__global__ __launch_bounds__(256) void k_wall_line(StepCtx c_arg, float *__restrict__ f, int k) {
+  const StepCtx &c = kernel_ctx();
+  const VsxWall &w = c.p.walls[k];
+  const int a = L::w_axis[w.loc];
+  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int co[3] = {0, 0, 0};
+  if constexpr (L::D == 2) {
+    if (j >= c.dims[1 - a]) return;
+    co[1 - a] = (int)j;
+  } else {
+    const int b0 = a == 0 ? 1 : 0, b1 = a == 2 ? 1 : 2;
+    if (j >= (int64_t)c.dims[b0] * c.dims[b1]) return;
+    co[b0] = (int)(j / c.dims[b1]);
+    co[b1] = (int)(j % c.dims[b1]);
+  }
+  co[a] = L::w_sign[w.loc] > 0 ? 0 : c.dims[a] - 1;
+  float fw[L::Q], fn[L::Q], fp[L::Q];
+  const uint32_t cell = cell_of<L>(c, co[0], co[1], co[2]);
+#pragma unroll
+  for (int i = 0; i < L::Q; ++i) {
+    fw[i] = f[(size_t)i * c.n + cell];
+    fn[i] = fp[i] = 0.f;
+  }
+  if (wall_needs_neighbor(w)) {
+    int cn[3];
+    inward<L>(w.loc, co, cn);
+    const uint32_t nb = cell_of<L>(c, cn[0], cn[1], cn[2]);
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i) fn[i] = f[(size_t)i * c.n + nb];
+  }
+  apply_wall<L>(w, wall_node<L>(c, w.loc, co), fw, fn, fp);
+#pragma unroll
+  for (int i = 0; i < L::Q; ++i) f[(size_t)i * c.n + cell] = fw[i];
+}
+
+// Walls of a pushed field, in list order (one launch per wall).
+template <class L>
+int walls_in_place(const StepCtx &c, float *f, hipStream_t s) {
+  for (int k = 0; k < c.p.n_walls; ++k) {
+    const int a = L::w_axis[c.p.walls[k].loc];
+    int64_t nodes = 1;
+    for (int b = 0; b < L::D; ++b)
+      if (b != a) nodes *= c.dims[b];
+    hipLaunchKernelGGL((k_wall_line<L>), dim3((unsigned)((nodes + 255) / 256)), dim3(256), 0, s, c, f, k);
+  }
+  VSX_CHECK_LAUNCH();
+  return VSX_OK;
+}
+
+// Whether a run can end with a pushed step: no wall reads pre-streaming populations.
+inline bool walls_allow_push(const VsxLbm &p) {
+  for (int k = 0; k < p.n_walls; ++k)
+    if (wall_needs_pre(p.walls[k])) return false;
+  return true;
+}
+
+// ------------------------------------------------------------------ launches
+Tiles make_tiles(const StepCtx &c);
+
+// Bulk blocks take two tiles (PAIR) for the 16-bit storages on grids of at
+// least kPairMinTiles tiles: there the streaming bounds the step and the pairs
+// double the bytes in flight.  On smaller grids single tiles keep the step
+// shorter; FP32 tiles stay single (no gain at 4096^2, a loss at 1024^2).
+constexpr int kPairMinTiles = 4096;
+
+template <class SRC, class DST, int COLL, bool PAIR, int MODE>
+int launch_fused_p(const StepCtx &c, const VsxIB2D &ib, const void *src, void *dst, int step, hipStream_t s,
+                   const Tiles &T) {
+  const size_t lds = 4 * (size_t)ib_atomic_lds_words(ib.win);
+  const int n_tiles = T.ntx * T.nty;
+  int grid = n_tiles;
+  if (PAIR) {
+    const int nw_max = ((ib.win + T.TX - 1) / T.TX + 1) * ((ib.win + T.TY - 1) / T.TY + 1);
+    grid = (n_tiles + nw_max + 1) / 2;
+    if (grid > n_tiles) grid = n_tiles;
+  }
+  timer_begin(s);
+  hipLaunchKernelGGL((k_viv_fused<SRC, DST, COLL, PAIR, MODE>), dim3(1 + grid), dim3(kTileCells), lds, s, c, ib,
+                     static_cast<const typename SRC::T *>(src), static_cast<typename DST::T *>(dst), step, T);
+  timer_end(s);
+  VSX_CHECK_LAUNCH();
+  return VSX_OK;
+}
+
+template <class SRC, class DST, int COLL, int MODE>
+int launch_fused_c(const StepCtx &c, const VsxIB2D &ib, const void *src, void *dst, int step, hipStream_t s) {
+  const Tiles T = make_tiles(c);
+  if constexpr (MODE == kModeFull && sizeof(typename SRC::T) == 2) {
+    if (T.ntx * T.nty >= kPairMinTiles) return launch_fused_p<SRC, DST, COLL, true, MODE>(c, ib, src, dst, step, s, T);
+  }
+  return launch_fused_p<SRC, DST, COLL, false, MODE>(c, ib, src, dst, step, s, T);
+}
+
+// storage ST of the work buffers; kModeFirst reads FP32, kModeLast writes FP32
+template <class ST, int MODE>
+int launch_fused_mode(const StepCtx &c, const VsxIB2D &ib, const void *src, void *dst, int step, hipStream_t s) {
+  using F32 = Store<VSX_STORE_F32>;
+  using SRC = std::conditional_t<MODE == kModeFirst, F32, ST>;
+  using DST = std::conditional_t<MODE == kModeLast, F32, ST>;
+  switch (c.p.collision) {
+    case VSX_COLL_BGK: return launch_fused_c<SRC, DST, VSX_COLL_BGK, MODE>(c, ib, src, dst, step, s);
+    case VSX_COLL_MRT: return launch_fused_c<SRC, DST, VSX_COLL_MRT, MODE>(c, ib, src, dst, step, s);
+    case VSX_COLL_KBC: return launch_fused_c<SRC, DST, VSX_COLL_KBC, MODE>(c, ib, src, dst, step, s);
+    default: return launch_fused_c<SRC, DST, VSX_COLL_REG, MODE>(c, ib, src, dst, step, s);
+  }
+}
+
+template <int MODE>
+int fused_mode_any(const StepCtx &c, const VsxIB2D &ib, const void *src, void *dst, int step, hipStream_t s) {
+  switch (c.p.storage) {
+    case VSX_STORE_F32: return launch_fused_mode<Store<VSX_STORE_F32>, MODE>(c, ib, src, dst, step, s);
+    case VSX_STORE_F16S: return launch_fused_mode<Store<VSX_STORE_F16S>, MODE>(c, ib, src, dst, step, s);
+    case VSX_STORE_BF16S: return launch_fused_mode<Store<VSX_STORE_BF16S>, MODE>(c, ib, src, dst, step, s);
+    default: return launch_fused_mode<Store<VSX_STORE_E3M12S>, MODE>(c, ib, src, dst, step, s);
+  }
+}
+
+// defined in the translation unit that instantiates each mode
+int fused_full(const StepCtx &c, const VsxIB2D &ib, const void *src, void *dst, int step, hipStream_t s);
+int fused_first(const StepCtx &c, const VsxIB2D &ib, const void *src, void *dst, int step, hipStream_t s);
+int fused_last(const StepCtx &c, const VsxIB2D &ib, const void *src, void *dst, int step, hipStream_t s);
+
+}  // namespace vsx

@@ -201,6 +201,36 @@ class VIVCase:
         return u
 
 
+class _TimeoutWatch:
+    """Sticky check of the bounded-wait time-out flag of fused steps without a
+    host sync: after each run the flag is copied asynchronously into pinned
+    host memory behind an event; the next run (or an explicit ``check()``)
+    raises if a completed copy shows a time-out."""
+
+    def __init__(self, flags):
+        self.flags = list(flags)            # int32 device scalars (views)
+        self.host = torch.zeros(len(self.flags), dtype=torch.int32, pin_memory=True)
+        self.event = None
+
+    def after_run(self):
+        for k, fl in enumerate(self.flags):
+            self.host[k:k + 1].copy_(fl, non_blocking=True)
+        self.event = torch.cuda.Event()
+        self.event.record()
+
+    def poll(self, wait=False):
+        if self.event is None:
+            return
+        if wait:
+            self.event.synchronize()
+        elif not self.event.query():
+            return
+        bad = [k for k in range(len(self.flags)) if int(self.host[k]) != 0]
+        if bad:
+            raise RuntimeError(f"fused VIV step: a tile of case(s) {bad} timed out waiting for the IB workgroup "
+                               "(the force window of that step was not applied)")
+
+
 class _Body2D:
     """Device buffers of one 2D immersed body and the VsxIB2D that points at them."""
 
@@ -251,6 +281,7 @@ class VIV2D:
         self.t_total = 0
         self.ib = self._b.ib
         self.ib.fused = int(bool(fused))
+        self._watch = _TimeoutWatch([self.win_origin[3:4]])
 
     def set_body(self, a=(0.0, 0.0), v=(0.0, 0.0), d=(0.0, 0.0)):
         """Reset the structure state (and the step counter of the body slots)."""
@@ -258,16 +289,20 @@ class VIV2D:
         self.t_total = 0
 
     def state(self):
+        self._watch.poll(wait=True)
         b = self.body[self.t_total & 1].cpu().numpy()
         return b[0:2].copy(), b[2:4].copy(), b[4:6].copy()   # a, v, d
 
     def check(self):
-        """Raise if a force-window tile of a fused step timed out waiting for the IB workgroup."""
-        if int(self.win_origin[3].item()) != 0:
-            raise RuntimeError("fused VIV step: window tile timed out waiting for the IB workgroup")
+        """Raise if a tile of a fused step timed out waiting for the IB workgroup
+        (waits for the runs queued so far).  ``run`` raises the same error, without
+        waiting, once a time-out of an earlier run is visible on the host."""
+        self._watch.after_run()
+        self._watch.poll(wait=True)
 
     def run(self, f, n_steps, out=None, history=True):
         """Advance n_steps; returns (f_out, d_hist [n,2], h_hist [n,2]) (histories FP64 on device)."""
+        self._watch.poll()
         f = f32(f)
         out = torch.empty_like(f) if out is None else out
         hd = hh = None
@@ -281,6 +316,7 @@ class VIV2D:
         L.call("vsx_viv2d_run", ctypes.byref(self.lbm.p), ctypes.byref(self.ib), L.ptr(f), L.ptr(out), L.ptr(wa),
                L.ptr(wb), int(n_steps))
         self.t_total += int(n_steps)
+        self._watch.after_run()
         return out, hd, hh
 
 
@@ -432,7 +468,10 @@ class BatchGraph:
         self._refs = refs     # keeps the captured buffers alive
 
     def launch(self):
+        batch = self._refs[2]
+        batch._watch.poll()
         L.call("vsx_graph_launch", self._h)
+        batch._watch.after_run()
 
     def __del__(self):
         if getattr(self, "_h", None) and L._LIB is not None:
@@ -489,6 +528,7 @@ class VIVBatch:
         L.check(L.lib().vsx_viv2d_batch_prepare(vc, len(cases), L.ptr(self.table), L.ptr(self.clock_t),
                                                 ctypes.byref(b)))
         self.batch = b
+        self._watch = _TimeoutWatch([bd.win_origin[3:4] for bd in self.bodies])
         dt = _STORE_DTYPE[storage]
         self._work = [torch.empty(self.total, dtype=dt, device=dev) for _ in range(2)]
 
@@ -523,17 +563,19 @@ class VIVBatch:
         return b[0:2].copy(), b[2:4].copy(), b[4:6].copy()
 
     def check(self):
-        for k, b in enumerate(self.bodies):
-            if int(b.win_origin[3].item()) != 0:
-                raise RuntimeError(f"case {k}: window tile timed out waiting for the IB workgroup")
+        """Raise if a tile of any case timed out waiting for its IB workgroup (waits for the queued work)."""
+        self._watch.after_run()
+        self._watch.poll(wait=True)
 
     def run(self, f, n_steps, out=None):
         f = f32(f)
         require(f.numel() == self.total, "f must be the packed batch buffer")
+        self._watch.poll()
         out = torch.empty_like(f) if out is None else out
         wa, wb = self._work
         L.call("vsx_viv2d_batch_run", ctypes.byref(self.batch), L.ptr(f), L.ptr(out), L.ptr(wa), L.ptr(wb),
                int(n_steps))
+        self._watch.after_run()
         return out
 
     def graph(self, f_in, f_out, n_steps):
