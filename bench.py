@@ -10,14 +10,15 @@ step; ``value`` is GLUPS = cells * steps / seconds / 1e9 with the state
 resident in HBM.
 
 Also reported on the same line:
-  roofline      : the fused step kernel's achieved algorithmic bandwidth
+  roofline      : the bulk pass kernel's achieved algorithmic bandwidth
                   (72 B per cell in FP32 / 36 B in 16-bit storage) from HIP events
-                  recorded around every launch in the timed region, vs 8 TB/s
+                  recorded around every launch of a second run, vs 8 TB/s
   variants      : FP16 / BF16 / E3M12 shifted-storage GLUPS of the same step,
-                  the step as two launches, 4096^2 in FP32 and 16-bit storage, the
+                  the step in its other launch forms (one launch per step, separate
+                  IB and bulk launches), 4096^2 in FP32 and 16-bit storage, the
                   3D sphere, a 64-case batch and the adjoint / forward ratio
-  cpu_baseline  : the oracle (NumPy FP32 restatement of the reference step),
-                  single-threaded on a bounded sample of the same workload
+  cpu_baseline  : the reference's step restated in C (oracle/c, float32, OpenMP
+                  over the host cores) on a bounded sample of the same workload
 """
 
 import argparse
@@ -30,10 +31,6 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per k_viv_fused launch on the 1024^2 FP32 workload, from rocprofv3 FETCH_SIZE / WRITE_SIZE
-# passes scaled by a 256 MiB copy (profiles/k_viv_fused_viv2d_1024x1024_f32_pmc.md); a profiler run
-# cannot be part of the timed bench itself.
-TRAFFIC_1024_F32 = 79692224
 
 
 def kernel_times(lib, n):
@@ -44,25 +41,32 @@ def kernel_times(lib, n):
     return np.array(ms[: got.value], dtype=np.float64)
 
 
-def run_gpu(args, storage, steps, warmup, fused=True):
+def run_gpu(args, storage, steps, warmup, mode="default"):
+    """mode: "default" (the VIV2D default: pipelined, IB kernel of the next step on a second
+    stream), "one_launch" or "separate"."""
     import torch
     from vivsim_amd import _lib, lbm, sim
 
     case = sim.VIVCase(nx=args.nx, ny=args.ny)
-    solver = sim.VIV2D(case, storage=storage, fused=fused)
+    kw = dict(default={}, one_launch=dict(pipelined=False), separate=dict(fused=False))[mode]
+    solver = sim.VIV2D(case, storage=storage, **kw)
     solver.set_body(v=(0.0, 1e-3 * case.U0))
     u = torch.as_tensor(case.initial_u(), dtype=torch.float32, device="cuda")
     f = lbm.get_equilibrium(torch.ones(case.nx, case.ny, device="cuda"), u)
     f, _, _ = solver.run(f, warmup, history=False)
+    # caller-owned output and histories, as a loop of runs would keep them
+    out = torch.empty_like(f)
+    hist = (torch.empty((steps, 2), dtype=torch.float64, device="cuda"),
+            torch.empty((steps, 2), dtype=torch.float64, device="cuda"))
     torch.cuda.synchronize()
     # throughput: the plain run, no instrumentation in the stream
     t0 = time.perf_counter()
-    out, hd, hh = solver.run(f, steps)
+    out, hd, hh = solver.run(f, steps, out=out, history=hist)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     assert torch.isfinite(out).all().item(), "non-finite state"
     solver.check()
-    # kernel duration: a second run with a HIP event pair around every fused launch
+    # kernel duration: a second run with a HIP event pair around every bulk launch
     lib = _lib.lib()
     n_timed = min(steps, 400)
     assert lib.vsx_timer_start(n_timed) == 0
@@ -72,12 +76,11 @@ def run_gpu(args, storage, steps, warmup, fused=True):
     solver.check()
     cells = case.nx * case.ny
     sec = t1 - t0
-    bulk_cells = case.nx * case.ny                  # every cell, wall columns included, is in the fused launch
     bpc = 72 if storage == "f32" else 36
     mean_ms = float(kt.mean())
     return dict(glups=cells * steps / sec / 1e9, ms_per_step=sec / steps * 1e3, kernel_ms=mean_ms,
-                kernel_launches=int(kt.size), bytes_per_launch=bpc * bulk_cells,
-                achieved_gbs=bpc * bulk_cells / (mean_ms * 1e-3) / 1e9,
+                kernel_launches=int(kt.size), bytes_per_launch=bpc * cells,
+                achieved_gbs=bpc * cells / (mean_ms * 1e-3) / 1e9, mode=solver.ib.fused,
                 lift_last=float(hh[-1, 1].item()), d_last=float(hd[-1, 1].item()))
 
 
@@ -162,26 +165,25 @@ def run_adjoint(steps):
 
 
 def run_cpu(args, seconds_budget):
-    """Oracle VIV step (float32, one thread) on the same 1024^2 workload."""
-    from threadpoolctl import threadpool_limits
-
+    """The reference's VIV step restated in C (oracle/c/viv_cpu.c, OpenMP over the
+    host cores OMP_NUM_THREADS allows) on the same 1024^2 workload, for a bounded
+    number of steps."""
+    from oracle import cpu
     from oracle import steps as S
 
     case = S.VIV2D(nx=args.nx, ny=args.ny)
+    run = cpu.VivRunner(case)
     f = case.initial_f(np.float32)
-    d, v, a = np.zeros(2), np.array([0.0, 1e-3 * case.U0]), np.zeros(2)
-    n = 0
-    with threadpool_limits(1):
-        t0 = time.perf_counter()
-        while True:
-            f, d, v, a, h = S.viv_step(case, f, d, v, a)
-            n += 1
-            if time.perf_counter() - t0 > seconds_budget or n >= args.cpu_max_steps:
-                break
-        sec = time.perf_counter() - t0
-    return dict(value=case.nx * case.ny * n / sec / 1e9, unit="GLUPS", cores=1, kind="port",
-                sample=f"{n} oracle VIV steps (NumPy float32, 1 thread) on the {args.nx}x{args.ny} workload, "
-                       f"{sec:.1f} s")
+    body = np.array([0.0, 0.0, 0.0, 1e-3 * case.U0, 0.0, 0.0])
+    run.run(f, 2, body)                                    # warm-up (first touch of the buffers)
+    n, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds_budget and n < args.cpu_max_steps:
+        run.run(f, 5, body)
+        n += 5
+    sec = time.perf_counter() - t0
+    return dict(value=case.nx * case.ny * n / sec / 1e9, unit="GLUPS", cores=cpu.threads(), kind="port",
+                sample=f"{n} VIV steps of the {args.nx}x{args.ny} workload by the C restatement of the reference "
+                       f"step (float32, OpenMP, {cpu.threads()} threads), {sec:.1f} s")
 
 
 def main():
@@ -192,8 +194,8 @@ def main():
     ap.add_argument("--nx", type=int, default=1024)
     ap.add_argument("--ny", type=int, default=1024)
     ap.add_argument("--storage", default="f32", choices=["f32", "f16s", "bf16s", "e3m12s"])
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--cpu-max-steps", type=int, default=60)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-max-steps", type=int, default=5000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=2000)
@@ -222,34 +224,49 @@ def main():
     torch.cuda.set_device(0)
 
     main_run = run_gpu(args, args.storage, args.steps, args.warmup)
+    print(f"headline: {main_run['glups']:.3f} GLUPS, {main_run['ms_per_step'] * 1e3:.2f} us / step, "
+          f"bulk kernel {main_run['kernel_ms'] * 1e3:.2f} us", file=sys.stderr, flush=True)
     variants = {}
+
+    def variant(name, fn):
+        """A variant that fails is recorded as such; it never loses the headline line."""
+        try:
+            variants[name] = fn()
+        except Exception as e:  # noqa: BLE001
+            variants[name] = {"error": f"{type(e).__name__}: {e}"}
+        print(f"variant {name}: {variants[name]}", file=sys.stderr, flush=True)
+
+    def short(r, **extra):
+        d = dict(glups=round(r["glups"], 3), ms_per_step=round(r["ms_per_step"], 5),
+                 kernel_us=round(r["kernel_ms"] * 1e3, 3), achieved_gbs=round(r["achieved_gbs"], 1))
+        d.update(extra)
+        return d
+
     if not args.no_variants:
         for st in ("f32", "f16s", "bf16s", "e3m12s"):
             if st != args.storage:
-                r = run_gpu(args, st, args.steps, args.warmup)
-                variants[st] = dict(glups=round(r["glups"], 3), ms_per_step=round(r["ms_per_step"], 5),
-                                    kernel_us=round(r["kernel_ms"] * 1e3, 3), achieved_gbs=round(r["achieved_gbs"], 1))
-        # the same FP32 step as two launches (IB kernel, then the bulk kernel)
-        r = run_gpu(args, args.storage, args.steps, args.warmup, fused=False)
-        variants["separate_launches"] = dict(glups=round(r["glups"], 3), ms_per_step=round(r["ms_per_step"], 5),
-                                             bulk_kernel_us=round(r["kernel_ms"] * 1e3, 3))
+                variant(st, lambda st=st: short(run_gpu(args, st, args.steps, args.warmup)))
+        # the same step in the other launch forms
+        variant("one_launch", lambda: short(run_gpu(args, args.storage, args.steps, args.warmup, "one_launch")))
+        variant("separate_launches", lambda: short(run_gpu(args, args.storage, args.steps, args.warmup, "separate")))
         # beyond the 256 MB last-level cache: 4096^2 (2 x 1.2 GB populations), same body
         big = argparse.Namespace(**{**vars(args), "nx": 4096, "ny": 4096})
-        r = run_gpu(big, "f32", 200, 20)
-        variants["grid_4096"] = dict(glups=round(r["glups"], 3), ms_per_step=round(r["ms_per_step"], 5),
-                                     kernel_us=round(r["kernel_ms"] * 1e3, 3), achieved_gbs=round(r["achieved_gbs"], 1),
-                                     frac=round(r["achieved_gbs"] / HBM_PEAK_GBS, 4))
-        # 16-bit storage where the streaming, not the IB workgroup, bounds the step
+        r32 = {}
+
+        def big_f32():
+            r32.update(run_gpu(big, "f32", 200, 20))
+            return short(r32, frac=round(r32["achieved_gbs"] / HBM_PEAK_GBS, 4))
+        variant("grid_4096", big_f32)
+        # 16-bit storage where the streaming, not the IB, bounds the step
         for st in ("f16s", "e3m12s"):
-            r16 = run_gpu(big, st, 200, 20)
-            variants[f"grid_4096_{st}"] = dict(glups=round(r16["glups"], 3), ms_per_step=round(r16["ms_per_step"], 5),
-                                               kernel_us=round(r16["kernel_ms"] * 1e3, 3),
-                                               achieved_gbs=round(r16["achieved_gbs"], 1),
-                                               frac=round(r16["achieved_gbs"] / HBM_PEAK_GBS, 4),
-                                               speedup_vs_f32=round(r16["glups"] / r["glups"], 3))
-        variants["sphere3d"] = run_sphere(100)
-        variants["batch64"] = run_batch(400)
-        variants["adjoint"] = run_adjoint(args.adjoint_steps)
+            def big16(st=st):
+                r16 = run_gpu(big, st, 200, 20)
+                return short(r16, frac=round(r16["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                             speedup_vs_f32=round(r16["glups"] / r32["glups"], 3) if r32 else None)
+            variant(f"grid_4096_{st}", big16)
+        variant("sphere3d", lambda: run_sphere(100))
+        variant("batch64", lambda: run_batch(400))
+        variant("adjoint", lambda: run_adjoint(args.adjoint_steps))
     cpu = None if args.no_cpu else run_cpu(args, args.cpu_seconds)
 
     line = {
@@ -270,8 +287,11 @@ def main():
                    "forcing": "guo", "kernel": "peskin_4pt", "storage": args.storage, "structure": "fp64 newmark"},
         "roofline": {"bound": "hbm", "achieved": round(main_run["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(main_run["achieved_gbs"] / HBM_PEAK_GBS, 4),
-                     "traffic": TRAFFIC_1024_F32 if (args.nx, args.ny, args.storage) == (1024, 1024, "f32") else None,
-                     "kernel": "k_viv_fused (IB workgroup + pull-stream/MRT/Guo tiles, one launch per step)",
+                     # HBM bytes per launch come from rocprofv3 FETCH_SIZE / WRITE_SIZE passes, which cannot
+                     # run inside this timed process; the measured figures are kept under profiles/
+                     "traffic": None,
+                     "kernel": "k_viv_fused (pull-stream + walls + MRT/Guo bulk pass; pipelined form: the IB "
+                               "kernel of the next step runs on a second stream)",
                      "kernel_us": round(main_run["kernel_ms"] * 1e3, 3),
                      "bytes_per_launch": main_run["bytes_per_launch"], "launches": main_run["kernel_launches"]},
         "variants": variants,

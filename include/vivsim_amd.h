@@ -110,13 +110,23 @@ typedef struct VsxIB2D {
   int32_t *win_origin; /* device int32 [8]: [0..1] window origin x, y of the last step;
                           [2] ready flag (= step0 + t + 1 once step t's force window is
                           written); [3] error flag (1 if a bounded wait ran out); [4..5]
-                          window-tile counters of the batched step; [6] = step0 + t + 1 once
-                          the IB workgroup of step t has pulled its window (the bulk tiles of
-                          the fused step wait for it); [7] reserved                          */
+                          window-tile counters (batched step; [4] also the hand-off tile
+                          counter of the pipelined step); [6] = step0 + t + 1 once the IB
+                          workgroup of step t has pulled its window (fused == 1: the bulk
+                          tiles wait for it) or once the bulk pass of step t has handed the
+                          cells around the window to the next IB kernel (fused == 2);
+                          [7] reserved                                                       */
   int32_t step0;       /* global index of the first step of this run (0 after a reset of the
                           body state, which must also zero win_origin)                       */
-  int32_t fused;       /* 1: one launch per step, the IB workgroup overlapped with the bulk
-                          pass; 0: separate IB and bulk kernels                              */
+  int32_t fused;       /* 0: separate IB and bulk kernels; 1: one launch per step, the IB
+                          workgroup overlapped with the bulk tiles; 2: pipelined, the IB
+                          kernel of step t + 1 (on a second stream) overlapped with the bulk
+                          pass of step t (needs `halo`, and the window >= 2 cells inside
+                          the grid)                                                          */
+  int32_t reserved;
+  float *halo;         /* device [2][9][win + 4][win + 4] FP32 (fused == 2): post-collision
+                          populations around the window, handed from the bulk pass of a step
+                          to the IB kernel of the next (half (step0 + t) & 1 by step t)     */
 } VsxIB2D;
 
 /* 3D immersed body held fixed in a D3Q19 VsxLbm step (flow past a sphere).
@@ -135,10 +145,15 @@ typedef struct VsxIB3D {
   const float *u_target;      /* device [M][3] target marker velocity                          */
   const float *ds_arr;        /* device [M] or NULL -> ds                                      */
   float ds;
-  int32_t reserved;
+  int32_t pipelined;          /* 1: the IB of step t + 1 runs on a second stream while the bulk
+                                 pass of step t streams (needs `halo`, the window 2+ cells inside
+                                 the grid, and walls without shared corners)                   */
   float *g_win;               /* device [3][cells]: force window of the last step              */
   float *work;                /* device scratch, vsx_ib3d_work_floats(M, cells) floats           */
   double *hist_h;             /* optional device [n_steps][3]: total marker force (FP64)        */
+  float *halo;                /* device [2][19][S0 + 2][S1 + 2][S2 + 2] FP32 (pipelined): the
+                                 post-collision populations around the window, handed from the
+                                 bulk pass of a step to the IB of the next                       */
 } VsxIB3D;
 
 /* One case of a batched 2D VIV sweep.  All cases of a batch advance together,
@@ -360,6 +375,40 @@ int vsx_ib3d_get_surface_area(const float *verts, int32_t nv, const int32_t *fac
                               void *stream);
 int vsx_ib3d_get_volume(const float *verts, int32_t nv, const int32_t *faces, int32_t nf, float *out, void *stream);
 int vsx_ib3d_get_ds(const float *verts, int32_t nv, const int32_t *faces, int32_t nf, float *out, void *stream);
+
+/* ------------------------------------------------------------------ diagnostics (post.py)
+ * u is a velocity field [D][NX][NY] (D = 2, nz = 1) or [D][NX][NY][NZ] (D = 3); derivatives are
+ * jnp.gradient's (unit spacing; central inside, one-sided first order at the ends). */
+enum { VSX_POST_GRADIENT = 0,                  /* post.py:18-31  out [D][D][n]: du_i / dx_j        */
+       VSX_POST_VORTICITY = 1,                 /* post.py:34-57  out [n] (2D) or [3][n] (3D)       */
+       VSX_POST_VORTICITY_MAGNITUDE = 2,       /* post.py:60-72                                    */
+       VSX_POST_DIVERGENCE = 3,                /* post.py:76-87                                    */
+       VSX_POST_STRAIN_RATE = 4,               /* post.py:90-99  out [D][D][n]                     */
+       VSX_POST_STRAIN_RATE_MAGNITUDE = 5,     /* post.py:102-111                                  */
+       VSX_POST_Q_CRITERION = 6,               /* post.py:176-191                                  */
+       VSX_POST_ENSTROPHY = 7,                 /* post.py:148-159                                  */
+       VSX_POST_VELOCITY_MAGNITUDE = 8,        /* post.py:6-15                                     */
+       VSX_POST_KINETIC_ENERGY = 9 };          /* post.py:114-122                                  */
+int vsx_post_field(const float *u, int32_t dim, int32_t nx, int32_t ny, int32_t nz, int32_t op, float *out,
+                   void *stream);
+/* post.py:125-133 (op = KINETIC_ENERGY) and 162-170 (op = ENSTROPHY): out[0] = mean over the
+ * cells, summed in FP64 in a fixed order; partial: device scratch of 1024 doubles */
+int vsx_post_mean(const float *u, int32_t dim, int32_t nx, int32_t ny, int32_t nz, int32_t op, double *partial,
+                  float *out, void *stream);
+/* post.py:136-145 */
+int vsx_post_pressure(const float *rho, int64_t n, float cs2, float *out, void *stream);
+/* examples/2d/vortex_induced_vibration.py:228-229 : cd[t], cl[t] = 2 h[t][0..1] / (D U0^2) */
+int vsx_force_coefficients(const double *h, int32_t n, double D, double U0, double *cd, double *cl, void *stream);
+
+/* ------------------------------------------------------------------ grid refinement (multigrid.py)
+ * dir: 0 left, 1 right, 2 up, 3 down; the fine grid has twice the coarse nodes along the
+ * interface.  Written in place into the destination field. */
+/* multigrid.py:58-98 */
+int vsx_mg_fine_to_coarse(const float *f_fine, int32_t nx_fine, int32_t ny_fine, float *f_coarse,
+                          int32_t nx_coarse, int32_t ny_coarse, int32_t dir, void *stream);
+/* multigrid.py:100-130 */
+int vsx_mg_coarse_to_fine(const float *f_coarse, int32_t nx_coarse, int32_t ny_coarse, float *f_fine,
+                          int32_t nx_fine, int32_t ny_fine, int32_t dir, void *stream);
 
 /* ------------------------------------------------------------------ structure (dyn.py), FP64 */
 /* dyn.py:5-51 : a, v, d, h [ndof]; m, k, c scalars (matrix == 0) or [ndof][ndof] (matrix != 0) */

@@ -1,6 +1,7 @@
 """Writes reference_signatures.json: the parameter lists (names, kinds and
 defaults) of every public function of the reference modules vivsim.lbm,
-vivsim.lbm3d, vivsim.ib, vivsim.ib3d and vivsim.dyn.
+vivsim.lbm3d, vivsim.ib, vivsim.ib3d, vivsim.dyn, vivsim.post and
+vivsim.multigrid.
 
 The reference is read as text with ``ast`` (nothing of it is imported or run;
 starred subscripts, Python >= 3.11 syntax, are unstarred first):
@@ -21,12 +22,24 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
+def _const(node):
+    """Value of a literal or of arithmetic on literals (e.g. ``cs2=1.0/3.0``)."""
+    if isinstance(node, ast.BinOp):
+        a, b = _const(node.left), _const(node.right)
+        ops = {ast.Add: lambda: a + b, ast.Sub: lambda: a - b, ast.Mult: lambda: a * b, ast.Div: lambda: a / b,
+               ast.Pow: lambda: a ** b}
+        return ops[type(node.op)]()
+    if isinstance(node, ast.UnaryOp) and isinstance(node.op, ast.USub):
+        return -_const(node.operand)
+    return ast.literal_eval(node)
+
+
 def _default(node):
     if node is None:
         return None
     if isinstance(node, ast.Name):
         return {"name": node.id}
-    return {"value": ast.literal_eval(node)}
+    return {"value": _const(node)}
 
 
 def _params(fn):
@@ -135,8 +148,9 @@ def main(ref_root):
     for pkg in ("lbm", "lbm3d", "ib", "ib3d"):
         init = _load(os.path.join(ref_root, "vivsim", pkg), pkg_root)
         out[pkg] = {name: signature(init, name, pkg_root) for name in exported(init)}
-    dyn = _load(os.path.join(ref_root, "vivsim", "dyn"), pkg_root)
-    out["dyn"] = {name: _params(fn) for name, fn in dyn.defs.items() if not name.startswith("_")}
+    for modname in ("dyn", "post", "multigrid"):
+        mod = _load(os.path.join(ref_root, "vivsim", modname), pkg_root)
+        out[modname] = {name: _params(fn) for name, fn in mod.defs.items() if not name.startswith("_")}
     dst = os.path.join(HERE, "reference_signatures.json")
     with open(dst, "w") as fh:
         json.dump(out, fh, indent=1, sort_keys=True)

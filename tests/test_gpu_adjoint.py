@@ -13,9 +13,10 @@ from tests.gpu_util import say, to_np
 
 pytestmark = pytest.mark.gpu
 
-GEOM = dict(nx=80, ny=64, D=10.0, cx=18.0, cy=32.0, n_markers=40, U0=0.05, Re=60.0, n_iter=3, pad=6,
+# the adjoint workload (512 x 256, D = 32, 256 markers, Re = 150) scaled to 128 x 64
+GEOM = dict(nx=128, ny=64, D=8.0, cx=32.0, cy=32.0, n_markers=64, U0=0.05, Re=150.0, n_iter=3, pad=6,
             dof=(False, True))
-T = 40
+T = 60
 V0 = (0.0, 1e-3)
 
 
@@ -42,22 +43,60 @@ def reference():
 
 
 def test_adjoint_matches_autograd(sim, reference):
+    """The hand-written reverse pass (FP32 fields, FP64 structure) against
+    float64 autograd of the restatement, for two checkpoint intervals; the
+    relative errors are printed and held to 1e-3."""
     c = sim.VIVCase(**GEOM)
-    for seg in (7, 40):
-        say(f"adjoint seg={seg}")
+    r = reference
+    for seg in (7, T):
         adj = sim.VIVAdjoint(c, T, seg=seg, v0=V0)
-        J, gd, gk, gu = adj.evaluate(reference["delta"], reference["K"], reference["U"])
+        J, gd, gk, gu = adj.evaluate(r["delta"], r["K"], r["U"])
         gd = to_np(gd)
-        r = reference
-        assert abs(J - r["J"]) <= 2e-4 * abs(r["J"]), (J, r["J"])
-        assert np.linalg.norm(gd - r["gd"]) <= 2e-2 * np.linalg.norm(r["gd"]), (gd[:4], r["gd"][:4])
-        assert abs(gk - r["gk"]) <= 2e-2 * abs(r["gk"]), (gk, r["gk"])
-        assert abs(gu - r["gu"]) <= 5e-2 * abs(r["gu"]) + 1e-9, (gu, r["gu"])
+        e = dict(J=abs(J - r["J"]) / abs(r["J"]), delta=np.linalg.norm(gd - r["gd"]) / np.linalg.norm(r["gd"]),
+                 K=abs(gk - r["gk"]) / abs(r["gk"]), u_in=abs(gu - r["gu"]) / abs(r["gu"]))
+        say(f"adjoint seg={seg} vs float64 autograd, relative errors: " +
+            ", ".join(f"{k} {v:.2e}" for k, v in e.items()))
+        assert e["J"] <= 1e-4 and e["delta"] <= 1e-3 and e["K"] <= 1e-3 and e["u_in"] <= 1e-3, e
 
 
 def test_adjoint_matches_finite_differences(sim, reference):
-    """Central differences of the HIP forward (FP32 field, FP64 displacement history): steps of
-    1 % of K and 0.02 cells for delta keep the difference well above the FP32 noise of J."""
+    """The adjoint against central differences with h = 1e-3 (relative for K
+    and u_in, in cells for delta) of the float64 restatement of the same run;
+    the relative errors are printed and held to 1e-3 (delta: relative to the
+    largest component).  Central differences of the FP32 HIP forward itself
+    are checked below at h = 1e-2, where the FP32 rounding of the field stays
+    below the objective's change."""
+    c = sim.VIVCase(**GEOM)
+    adj = sim.VIVAdjoint(c, T, seg=8, v0=V0)
+    d0, k0, u0 = reference["delta"], reference["K"], reference["U"]
+    J, gd, gk, gu = adj.evaluate(d0, k0, u0)
+    gd = to_np(gd)
+    oc = S.VIV2D(**GEOM)
+    m = VT.VIVModel(oc)
+    f0 = torch.tensor(oc.initial_f(np.float64))
+
+    def Jof(dl, k, u):
+        with torch.no_grad():
+            return m.objective(f0, T, torch.tensor(dl), torch.tensor(k, dtype=torch.float64),
+                               torch.tensor(u, dtype=torch.float64), v0=V0)[0].item()
+
+    h = 1e-3
+    fd_k = (Jof(d0, k0 * (1 + h), u0) - Jof(d0, k0 * (1 - h), u0)) / (2 * h * k0)
+    fd_u = (Jof(d0, k0, u0 * (1 + h)) - Jof(d0, k0, u0 * (1 - h))) / (2 * h * u0)
+    errs = dict(K=abs(fd_k - gk) / abs(gk), u_in=abs(fd_u - gu) / abs(gu))
+    for i in (3, 17, 40):
+        e = np.zeros_like(d0)
+        e[i] = h
+        fd = (Jof(d0 + e, k0, u0) - Jof(d0 - e, k0, u0)) / (2 * h)
+        errs[f"delta_{i}"] = abs(fd - gd[i]) / np.abs(gd).max()
+    say("adjoint vs central differences (h = 1e-3, float64 restatement), relative errors: " +
+        ", ".join(f"{k} {v:.2e}" for k, v in errs.items()))
+    assert all(v <= 1e-3 for v in errs.values()), errs
+
+
+def test_adjoint_matches_finite_differences_of_the_gpu_forward(sim, reference):
+    """Central differences of the HIP forward run (FP32 field, FP64 displacement
+    history and objective) with h = 1e-2 against the adjoint."""
     c = sim.VIVCase(**GEOM)
     adj = sim.VIVAdjoint(c, T, seg=8, v0=V0)
     d0, k0, u0 = reference["delta"], reference["K"], reference["U"]
@@ -65,12 +104,16 @@ def test_adjoint_matches_finite_differences(sim, reference):
     gd = to_np(gd)
     h = 1e-2
     fd_k = (adj.objective(d0, k0 * (1 + h), u0) - adj.objective(d0, k0 * (1 - h), u0)) / (2 * h * k0)
-    assert abs(fd_k - gk) <= 5e-2 * abs(gk), (fd_k, gk)
-    for i in (3, 17):
+    fd_u = (adj.objective(d0, k0, u0 * (1 + h)) - adj.objective(d0, k0, u0 * (1 - h))) / (2 * h * u0)
+    errs = dict(K=abs(fd_k - gk) / abs(gk), u_in=abs(fd_u - gu) / abs(gu))
+    for i in (3, 17, 40):
         e = np.zeros_like(d0)
-        e[i] = 2e-2
-        fd = (adj.objective(d0 + e, k0, u0) - adj.objective(d0 - e, k0, u0)) / (2 * 2e-2)
-        assert abs(fd - gd[i]) <= 5e-2 * np.abs(gd).max(), (i, fd, gd[i])
+        e[i] = h
+        fd = (adj.objective(d0 + e, k0, u0) - adj.objective(d0 - e, k0, u0)) / (2 * h)
+        errs[f"delta_{i}"] = abs(fd - gd[i]) / np.abs(gd).max()
+    say("adjoint vs central differences of the FP32 GPU forward (h = 1e-2), relative errors: " +
+        ", ".join(f"{k} {v:.2e}" for k, v in errs.items()))
+    assert all(v <= 2e-2 for v in errs.values()), errs
 
 
 def test_autograd_function(sim, reference):
@@ -81,5 +124,5 @@ def test_autograd_function(sim, reference):
     U = torch.tensor(reference["U"], dtype=torch.float64, requires_grad=True)
     J = sim.VIVObjective.apply(delta, K, U, adj)
     (3.0 * J).backward()
-    assert np.allclose(delta.grad.numpy(), 3.0 * reference["gd"], rtol=0, atol=6e-2 * np.abs(reference["gd"]).max())
-    assert abs(K.grad.item() - 3.0 * reference["gk"]) <= 6e-2 * abs(3.0 * reference["gk"])
+    assert np.allclose(delta.grad.numpy(), 3.0 * reference["gd"], rtol=0, atol=2e-3 * np.abs(reference["gd"]).max())
+    assert abs(K.grad.item() - 3.0 * reference["gk"]) <= 2e-3 * abs(3.0 * reference["gk"])

@@ -190,19 +190,22 @@ def small_case(cls, **kw):
                                      dict(cx=12.0, n_markers=48), dict(cx=150.0, cy=8.0), dict(n_markers=600)],
                          ids=["mrt_guo", "kbc_edm", "reg_edm", "bgk_guo_2dof_cos", "p3_nebb", "window_on_inlet",
                               "window_on_outlet_clamped", "dense_markers"])
-@pytest.mark.parametrize("fused", [True, False], ids=["fused", "separate"])
-def test_viv_matches_oracle(sim, variant, fused):
-    """dense_markers: 600 markers give 9600 stencil entries, more than the
-    segmented spread of the IB workgroup holds (8 per thread), so the binned
-    gather spreads them."""
+@pytest.mark.parametrize("mode", ["pipelined", "one_launch", "separate"])
+def test_viv_matches_oracle(sim, variant, mode):
+    """Every launch form: the pipelined step (IB kernel of the next step on a
+    second stream; windows 2+ cells inside the grid only), the one-launch step
+    and the separate IB and bulk launches.  dense_markers: 600 markers, one
+    marker per thread of the IB workgroup (more than half its threads)."""
     oc = small_case(S.VIV2D, **variant)
     pc = small_case(sim.VIVCase, **variant)
+    if mode == "pipelined" and (pc.ib_x0 < 3 or pc.ib_y0 < 3 or pc.ib_x0 + pc.ib_size + 3 > pc.nx):
+        pytest.skip("the pipelined step needs the window 2+ cells inside the grid")
     steps = 40
     f0 = oc.initial_f(np.float64)
     v0 = np.array([0.0, 1e-3])
-    solver = sim.VIV2D(pc, fused=fused)
+    solver = sim.VIV2D(pc, fused=mode != "separate", pipelined=mode == "pipelined")
     solver.set_body(v=v0)
-    say(f"viv {variant} fused={fused}")
+    say(f"viv {variant} {mode}")
     got, hd, hh = solver.run(dev(f0), steps)
     solver.check()
     f, d, v, a, dh, hhist = S.viv_run(oc, f0, steps, v=v0)

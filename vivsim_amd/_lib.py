@@ -53,7 +53,8 @@ class VsxIB2D(ctypes.Structure):
                 ("mass", c_double), ("stiffness", c_double), ("damping", c_double), ("area", c_double),
                 ("dt", c_double), ("gamma", c_double), ("beta", c_double),
                 ("body", c_void_p), ("g_win", c_void_p), ("hist_d", c_void_p), ("hist_h", c_void_p),
-                ("win_origin", c_void_p), ("step0", c_int32), ("fused", c_int32)]
+                ("win_origin", c_void_p), ("step0", c_int32), ("fused", c_int32),
+                ("reserved", c_int32), ("halo", c_void_p)]
 
 
 class VsxVivCase(ctypes.Structure):
@@ -76,8 +77,8 @@ class VsxAdj2D(ctypes.Structure):
 class VsxIB3D(ctypes.Structure):
     _fields_ = [("n_markers", c_int32), ("n_iter", c_int32), ("origin", c_int32 * 3), ("win", c_int32 * 3),
                 ("W", c_void_p), ("I", c_void_p), ("cell_ptr", c_void_p), ("cell_slot", c_void_p),
-                ("u_target", c_void_p), ("ds_arr", c_void_p), ("ds", c_float), ("reserved", c_int32),
-                ("g_win", c_void_p), ("work", c_void_p), ("hist_h", c_void_p)]
+                ("u_target", c_void_p), ("ds_arr", c_void_p), ("ds", c_float), ("pipelined", c_int32),
+                ("g_win", c_void_p), ("work", c_void_p), ("hist_h", c_void_p), ("halo", c_void_p)]
 
 
 _CT = {"p": c_void_p, "i": c_int32, "l": c_int64, "f": c_float, "d": c_double}
@@ -114,6 +115,12 @@ _SIGS = {
     "vsx_graph_launch": "pp",
     "vsx_graph_destroy": "p",
     "vsx_viv2d_adjoint": "ppppppp p",
+    "vsx_post_field": "piiiiip p",
+    "vsx_post_mean": "piiiipp p",
+    "vsx_post_pressure": "plfp p",
+    "vsx_force_coefficients": "piddpp p",
+    "vsx_mg_fine_to_coarse": "piipiii p",
+    "vsx_mg_coarse_to_fine": "piipiii p",
 }
 for _pfx in ("d2q9", "d3q19"):
     _SIGS.update({

@@ -595,7 +595,7 @@ int viv_adjoint(const VsxLbm *p, const VsxIB2D *ib, const VsxAdj2D *a, const flo
   for (int sg = 0; sg < nseg; ++sg) {
     const int t0 = sg * K, nn = std::min(K, T - t0);
     w.step0 = t0;
-    w.fused = 1;
+    w.fused = w.halo ? 2 : 1;   // the forward pass as vsx_viv2d_run runs it (pipelined when it can be)
     w.g_win = ib->g_win;
     w.hist_d = a->hist_d + 2 * (size_t)t0;
     w.hist_h = nullptr;

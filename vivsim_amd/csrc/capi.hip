@@ -1,6 +1,7 @@
 // Library-level entry points: version, error text, device sync.
 #include <cstdarg>
 #include <cstdio>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -54,6 +55,25 @@ void timer_end(hipStream_t s) {
     (void)hipEventRecord(g_timer.b[g_timer.n], s);
     ++g_timer.n;
   }
+}
+
+constexpr int kMaxDevices = 64;
+PipeRes g_pipe[kMaxDevices];
+std::mutex g_pipe_mu;
+
+int pipe_res(PipeRes **out) {
+  int dev = 0;
+  VSX_HIP(hipGetDevice(&dev));
+  VSX_REQUIRE(dev >= 0 && dev < kMaxDevices, "device index %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_pipe_mu);
+  PipeRes &r = g_pipe[dev];
+  if (!r.s) {
+    VSX_HIP(hipStreamCreateWithFlags(&r.s, hipStreamNonBlocking));
+    VSX_HIP(hipEventCreateWithFlags(&r.fork, hipEventDisableTiming));
+    VSX_HIP(hipEventCreateWithFlags(&r.join, hipEventDisableTiming));
+  }
+  *out = &r;
+  return VSX_OK;
 }
 
 }  // namespace vsx

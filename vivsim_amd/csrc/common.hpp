@@ -19,6 +19,16 @@ int fail(int code, const char *fmt, ...);
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Second stream of the pipelined steps (default priority, non-blocking) and the
+// two events that fork it from and join it to the caller's stream: one set per
+// device, created on first use and kept for the process.  Calls that use it
+// from several host threads at once on one device are not supported.
+struct PipeRes {
+  hipStream_t s = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+};
+int pipe_res(PipeRes **out);
+
 // Checks the launch (and any earlier asynchronous error) of the kernels just queued.
 #define VSX_CHECK_LAUNCH()                                                              \
   do {                                                                                  \
@@ -59,6 +69,7 @@ struct Store<VSX_STORE_F32> {
   __device__ static inline float cvt(T r, float) { return r; }
   __device__ static inline float ld(const T *p, uint32_t i, float) { return p[i]; }
   __device__ static inline void st(T *p, uint32_t i, float v, float) { p[i] = v; }
+  __device__ static inline float rt(float v, float) { return v; }   // value as read back after a store
 };
 
 template <>
@@ -67,6 +78,7 @@ struct Store<VSX_STORE_F16S> {
   __device__ static inline float cvt(T r, float w) { return (float)r + w; }
   __device__ static inline float ld(const T *p, uint32_t i, float w) { return (float)p[i] + w; }
   __device__ static inline void st(T *p, uint32_t i, float v, float w) { p[i] = (_Float16)(v - w); }
+  __device__ static inline float rt(float v, float w) { return (float)(_Float16)(v - w) + w; }
 };
 
 __device__ inline float bf16_to_f32(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
@@ -83,6 +95,7 @@ struct Store<VSX_STORE_BF16S> {
   __device__ static inline float cvt(T r, float w) { return bf16_to_f32(r) + w; }
   __device__ static inline float ld(const T *p, uint32_t i, float w) { return bf16_to_f32(p[i]) + w; }
   __device__ static inline void st(T *p, uint32_t i, float v, float w) { p[i] = f32_to_bf16(v - w); }
+  __device__ static inline float rt(float v, float w) { return bf16_to_f32(f32_to_bf16(v - w)) + w; }
 };
 
 // E3M12: 16 bits = sign | 15-bit magnitude code r, decoded as
@@ -109,6 +122,7 @@ struct Store<VSX_STORE_E3M12S> {
   __device__ static inline float cvt(T r, float w) { return e3m12_to_f32(r) + w; }
   __device__ static inline float ld(const T *p, uint32_t i, float w) { return e3m12_to_f32(p[i]) + w; }
   __device__ static inline void st(T *p, uint32_t i, float v, float w) { p[i] = f32_to_e3m12(v - w); }
+  __device__ static inline float rt(float v, float w) { return e3m12_to_f32(f32_to_e3m12(v - w)) + w; }
 };
 
 inline size_t store_bytes(int s) { return s == VSX_STORE_F32 ? 4 : 2; }
@@ -128,7 +142,10 @@ __device__ __forceinline__ void st_agent(float *p, float v) {
 __device__ __forceinline__ float ld_agent(const float *p) {
   return __hip_atomic_load(const_cast<float *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-// wait until every memory operation of this thread has completed
-__device__ __forceinline__ void wait_mem() { __builtin_amdgcn_s_waitcnt(0); }
+// Wait until every vector memory operation of this thread has completed (its
+// agent-scope stores have reached the coherent level).  The memory clobber
+// keeps the compiler from moving memory operations across the wait: a
+// hand-off relies on this completion wait, not on release / acquire ordering.
+__device__ __forceinline__ void wait_mem() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
 }  // namespace vsx

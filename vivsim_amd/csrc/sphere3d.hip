@@ -13,6 +13,18 @@
 // touch each window cell, in ascending marker order -- deterministic and free
 // of atomics.  The last spread of the reference loop (its interpolation only
 // updates ub, which is not used afterwards) is not evaluated.
+//
+// Pipelined form (ib.pipelined): the IB kernels of step t + 1 run on a second
+// stream while the bulk pass of step t streams.  The bulk blocks that cover
+// the window +- 1 cell write their new populations (as read back from
+// storage) into the FP32 hand-off buffer ib.halo (two halves by step parity)
+// with st_agent and count themselves; the last one raises the hand-off flag,
+// which the first IB kernel of the next step waits for before it pulls the
+// window velocity from the buffer.  The last IB kernel writes the force
+// window with st_agent and a one-thread kernel raises the ready flag; the bulk
+// blocks that intersect the window wait for it and read the force window with
+// ld_agent.  Flags: work words [4] ready, [5] hand-off counter, [6] hand-off
+// flag, [7] error (1: a bounded wait ran out).
 #include "ib_common.hpp"
 #include "step_impl.hpp"
 
@@ -22,6 +34,7 @@ constexpr int kS3 = 64;   // stencil points per marker (4 x 4 x 4)
 
 struct IB3DDev {
   int32_t *origin;   // [4] window origin (read by the bulk kernel)
+  int32_t *flags;    // [4] pipelined form: ready, hand-off counter, hand-off flag, error
   float *u_win, *acc, *ub, *F, *dF, *g_win;
   const float *W;
   const int32_t *I, *cell_ptr, *cell_slot;
@@ -33,14 +46,15 @@ struct IB3DDev {
   int64_t cells;
 };
 
-int64_t work_floats(int32_t m, int64_t cells) { return 4 + 2 * 3 * cells + 3 * 3 * (int64_t)m; }
+int64_t work_floats(int32_t m, int64_t cells) { return 8 + 2 * 3 * cells + 3 * 3 * (int64_t)m; }
 
 IB3DDev make_dev(const VsxIB3D &ib) {
   IB3DDev d;
   d.cells = (int64_t)ib.win[0] * ib.win[1] * ib.win[2];
   float *w = ib.work;
   d.origin = reinterpret_cast<int32_t *>(w);
-  d.u_win = w + 4;
+  d.flags = reinterpret_cast<int32_t *>(w + 4);
+  d.u_win = w + 8;
   d.acc = d.u_win + 3 * d.cells;
   d.ub = d.acc + 3 * d.cells;
   d.F = d.ub + 3 * (int64_t)ib.n_markers;
@@ -144,7 +158,7 @@ __device__ __forceinline__ double wave_sum3(double v) {
 // mf[m][a] * W[slot] (ib/stencil.py:81-110 as a gather).  With hist, block 0
 // also writes the total marker force sum_m (-F_m) in FP64 (dyn.py:126-136).
 __global__ __launch_bounds__(256) void k_spread3d(IB3DDev d, const float *__restrict__ mf, float *__restrict__ out,
-                                                  double *hist) {
+                                                  double *hist, bool agent) {
   const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (k < d.cells) {
     float s0 = 0.f, s1 = 0.f, s2 = 0.f;
@@ -171,9 +185,15 @@ __global__ __launch_bounds__(256) void k_spread3d(IB3DDev d, const float *__rest
       s1 += mf[3 * m + 1] * w;
       s2 += mf[3 * m + 2] * w;
     }
-    out[k] = s0;
-    out[d.cells + k] = s1;
-    out[2 * d.cells + k] = s2;
+    if (agent) {   // read by the bulk pass of another stream
+      st_agent(&out[k], s0);
+      st_agent(&out[d.cells + k], s1);
+      st_agent(&out[2 * d.cells + k], s2);
+    } else {
+      out[k] = s0;
+      out[d.cells + k] = s1;
+      out[2 * d.cells + k] = s2;
+    }
   }
   if (hist && blockIdx.x == 0) {
     __shared__ double red[3][4];
@@ -186,6 +206,121 @@ __global__ __launch_bounds__(256) void k_spread3d(IB3DDev d, const float *__rest
     __syncthreads();
     if (threadIdx.x == 0)
       for (int a = 0; a < 3; ++a) hist[a] = red[a][0] + red[a][1] + red[a][2] + red[a][3];
+  }
+}
+
+// ------------------------------------------------------------------ pipelined form
+// Bounded wait of thread 0 until flags[k] >= target; on time-out flags[3] = 1.
+__device__ __forceinline__ void spin3d(int32_t *flags, int k, int target) {
+  int spins = 0;
+  while (__hip_atomic_load(&flags[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    __builtin_amdgcn_s_sleep(2);
+    if (++spins > (1 << 22)) {
+      __hip_atomic_store(&flags[3], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      break;
+    }
+  }
+}
+
+// Window velocity of step `step` pulled from the hand-off buffer written by the
+// bulk pass of step - 1 (the box of the window +- 1 cell, origin org - 1).
+__global__ __launch_bounds__(256) void k_win_u3d_halo(IB3DDev d, const float *__restrict__ halo, int step) {
+  if (threadIdx.x == 0) spin3d(d.flags, 2, step);
+  __syncthreads();
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= d.cells) return;
+  const int E0 = d.S[0] + 2, E1 = d.S[1] + 2, E2 = d.S[2] + 2;
+  const int64_t EE = (int64_t)E0 * E1 * E2;
+  const float *h = halo + (size_t)((step - 1) & 1) * 19 * EE;
+  const int r2 = (int)(k % d.S[2]) + 1;
+  const int r1 = (int)((k / d.S[2]) % d.S[1]) + 1;
+  const int r0 = (int)(k / ((int64_t)d.S[2] * d.S[1])) + 1;
+  float f[19], rho, u[3];
+#pragma unroll
+  for (int q = 0; q < 19; ++q)
+    f[q] = ld_agent(&h[q * EE + ((int64_t)(r0 - D3Q19::cx[q]) * E1 + (r1 - D3Q19::cy[q])) * E2 + (r2 - D3Q19::cz[q])]);
+  macroscopic<D3Q19>(f, rho, u);
+  for (int a = 0; a < 3; ++a) d.u_win[a * d.cells + k] = u[a];
+}
+
+// ready flag of step `step`: the force window (written with st_agent by the
+// last spread of this stream) is complete
+__global__ void k_publish3d(IB3DDev d, int step) {
+  __hip_atomic_store(&d.flags[0], step + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// body force of a cell in the window of the pipelined form (the force window
+// comes from the other stream: agent-coherent loads)
+__device__ __forceinline__ void window_force3d(const IB3DDev &d, const int (&co)[3], float (&g)[3]) {
+  const int r0 = co[0] - d.org[0], r1 = co[1] - d.org[1], r2 = co[2] - d.org[2];
+  g[0] = g[1] = g[2] = 0.f;
+  if (r0 >= 0 && r0 < d.S[0] && r1 >= 0 && r1 < d.S[1] && r2 >= 0 && r2 < d.S[2]) {
+    const int64_t k = ((int64_t)r0 * d.S[1] + r1) * d.S[2] + r2;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) g[a] = ld_agent(&d.g_win[a * d.cells + k]);
+  }
+}
+
+// Bulk pass of the pipelined form: the 3D pull-scheme step (k_bulk) where the
+// blocks that intersect the window wait for the ready flag of this step, and
+// the blocks that touch the window +- 1 cell hand their cells to the next
+// step's IB (except in the last step of a run, LAST).  Grid as k_bulk: x =
+// blockIdx.z, y = blockIdx.y, z from blockIdx.x; PHASE COLLIDE reads the
+// caller's post-streaming field in place (no walls).
+template <int COLL, int PHASE, class SRC, class DST>
+__global__ __launch_bounds__(256) void k_bulk3d_pipe(StepCtx c_arg, IB3DDev d, const typename SRC::T *__restrict__ src,
+                                                     typename DST::T *__restrict__ dst, float *__restrict__ halo,
+                                                     int step, int n_halo_blocks, bool last) {
+  const StepCtx &c = kernel_ctx();
+  const int z0 = blockIdx.x * blockDim.x, z1 = z0 + blockDim.x;
+  const int co[3] = {(int)blockIdx.z, (int)blockIdx.y, z0 + (int)threadIdx.x};
+  const bool valid = co[2] < c.dims[2];
+  auto in_box = [&](int lo_pad, int hi_pad) {
+    return co[0] >= d.org[0] - lo_pad && co[0] < d.org[0] + d.S[0] + hi_pad && co[1] >= d.org[1] - lo_pad &&
+           co[1] < d.org[1] + d.S[1] + hi_pad && z0 < d.org[2] + d.S[2] + hi_pad && z1 > d.org[2] - lo_pad;
+  };
+  const bool win_block = in_box(0, 0), halo_block = in_box(1, 1);
+  if (win_block) {
+    if (threadIdx.x == 0) spin3d(d.flags, 0, step + 1);
+    __syncthreads();
+  }
+  float f[19];
+  const uint32_t cell = valid ? cell_of<D3Q19>(c, co[0], co[1], co[2]) : 0;
+  if (valid) {
+    if constexpr (PHASE == VSX_PHASE_COLLIDE) {
+      local<D3Q19, SRC>(c, src, cell, f);
+    } else {
+      const int cls = c.p.n_walls > 0 ? wall_class<D3Q19>(c, co) : -1;
+      if (cls >= 0) single_wall_state<D3Q19, SRC>(c, src, co, cls, f);
+      else pull<D3Q19, SRC>(c, src, co[0], co[1], co[2], f);
+    }
+    float rho, u[3], g[3];
+    macroscopic<D3Q19>(f, rho, u);
+    window_force3d(d, co, g);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) g[a] += c.p.g_uniform[a] + (c.p.g ? c.p.g[(size_t)a * c.n + cell] : 0.f);
+    collide_force<D3Q19, COLL>(f, rho, u, g, c.p.forcing, c.p.omega, c.p.mrt_rates);
+    store_cell<D3Q19, DST>(c, dst, cell, f);
+  }
+  if (halo_block && !last) {
+    const int E0 = d.S[0] + 2, E1 = d.S[1] + 2, E2 = d.S[2] + 2;
+    const int64_t EE = (int64_t)E0 * E1 * E2;
+    const int a = co[0] - (d.org[0] - 1), b = co[1] - (d.org[1] - 1), e = co[2] - (d.org[2] - 1);
+    float *h = halo + (size_t)(step & 1) * 19 * EE;
+    if (valid && a >= 0 && a < E0 && b >= 0 && b < E1 && e >= 0 && e < E2) {
+      const int64_t k = ((int64_t)a * E1 + b) * E2 + e;
+#pragma unroll
+      for (int q = 0; q < 19; ++q) st_agent(&h[q * EE + k], DST::rt(f[q], c.shift[q]));
+    }
+    wait_mem();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int before = __hip_atomic_fetch_add(&d.flags[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (before == n_halo_blocks - 1) {
+        __hip_atomic_store(&d.flags[1], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&d.flags[2], step + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
   }
 }
 
@@ -272,17 +407,63 @@ void win_u_any(const StepCtx &c, const IB3DDev &d, const void *src, bool first, 
   }
 }
 
-int ib3d_phase(const StepCtx &c, const IB3DDev &d, const void *src, bool first, double *hist, hipStream_t s) {
-  win_u_any(c, d, src, first, s);
+// The IB of one step.  pipe_step >= 0 (pipelined form): the window velocity of a
+// step after the first comes from the hand-off buffer, the force window is
+// written with st_agent and the ready flag raised for step pipe_step.
+int ib3d_phase(const StepCtx &c, const IB3DDev &d, const void *src, bool first, double *hist, hipStream_t s,
+               const float *halo = nullptr, int pipe_step = -1) {
   const unsigned gm = (unsigned)((d.M + 3) / 4), gc = (unsigned)((d.cells + 255) / 256);   // k_mark3d: 4 markers / block
+  if (pipe_step > 0 && !first) hipLaunchKernelGGL(k_win_u3d_halo, dim3(gc), dim3(256), 0, s, d, halo, pipe_step);
+  else win_u_any(c, d, src, first, s);
   hipLaunchKernelGGL(k_mark3d, dim3(gm), dim3(256), 0, s, d, 0);
   for (int it = 1; it < d.n_iter; ++it) {
-    hipLaunchKernelGGL(k_spread3d, dim3(gc), dim3(256), 0, s, d, (const float *)d.dF, d.acc, (double *)nullptr);
+    hipLaunchKernelGGL(k_spread3d, dim3(gc), dim3(256), 0, s, d, (const float *)d.dF, d.acc, (double *)nullptr, false);
     hipLaunchKernelGGL(k_mark3d, dim3(gm), dim3(256), 0, s, d, it);
   }
-  hipLaunchKernelGGL(k_spread3d, dim3(gc), dim3(256), 0, s, d, (const float *)d.F, d.g_win, hist);
+  hipLaunchKernelGGL(k_spread3d, dim3(gc), dim3(256), 0, s, d, (const float *)d.F, d.g_win, hist, pipe_step >= 0);
+  if (pipe_step >= 0) hipLaunchKernelGGL(k_publish3d, dim3(1), dim3(1), 0, s, d, pipe_step);
   VSX_CHECK_LAUNCH();
   return VSX_OK;
+}
+
+template <int COLL, int PHASE, class SRC, class DST>
+void launch_bulk3d_pipe(const StepCtx &c, const IB3DDev &d, const void *src, void *dst, float *halo, int step,
+                        int n_halo, bool last, hipStream_t s) {
+  dim3 grid, block;
+  bulk_grid<D3Q19>(c, grid, block);
+  timer_begin(s);
+  hipLaunchKernelGGL((k_bulk3d_pipe<COLL, PHASE, SRC, DST>), grid, block, 0, s, c, d,
+                     static_cast<const typename SRC::T *>(src), static_cast<typename DST::T *>(dst), halo, step,
+                     n_halo, last);
+  timer_end(s);
+}
+
+template <int PHASE, class SRC, class DST>
+void bulk3d_pipe_coll(const StepCtx &c, const IB3DDev &d, const void *src, void *dst, float *halo, int step,
+                      int n_halo, bool last, hipStream_t s) {
+  switch (c.p.collision) {
+    case VSX_COLL_BGK: return launch_bulk3d_pipe<VSX_COLL_BGK, PHASE, SRC, DST>(c, d, src, dst, halo, step, n_halo, last, s);
+    case VSX_COLL_MRT: return launch_bulk3d_pipe<VSX_COLL_MRT, PHASE, SRC, DST>(c, d, src, dst, halo, step, n_halo, last, s);
+    case VSX_COLL_KBC: return launch_bulk3d_pipe<VSX_COLL_KBC, PHASE, SRC, DST>(c, d, src, dst, halo, step, n_halo, last, s);
+    default: return launch_bulk3d_pipe<VSX_COLL_REG, PHASE, SRC, DST>(c, d, src, dst, halo, step, n_halo, last, s);
+  }
+}
+
+template <class ST>
+void bulk3d_pipe_any(const StepCtx &c, const IB3DDev &d, const void *src, void *dst, float *halo, int step,
+                     int n_halo, bool first, bool last, hipStream_t s) {
+  using F32 = Store<VSX_STORE_F32>;
+  if (first) bulk3d_pipe_coll<VSX_PHASE_COLLIDE, F32, ST>(c, d, src, dst, halo, step, n_halo, last, s);
+  else bulk3d_pipe_coll<VSX_PHASE_FULL, ST, ST>(c, d, src, dst, halo, step, n_halo, last, s);
+}
+
+// blocks of the bulk grid that touch the window +- 1 cell
+int64_t halo_blocks(const StepCtx &c, const IB3DDev &d) {
+  dim3 grid, block;
+  bulk_grid<D3Q19>(c, grid, block);
+  const int64_t bd = block.x;
+  const int64_t zb0 = (d.org[2] - 1) / bd, zb1 = (d.org[2] + d.S[2]) / bd;
+  return (int64_t)(d.S[0] + 2) * (d.S[1] + 2) * (zb1 - zb0 + 1);
 }
 
 int validate_ib3d(const VsxLbm *p, const VsxIB3D *ib) {
@@ -317,6 +498,39 @@ int sphere_run(const VsxLbm *p, const VsxIB3D *ib, const float *f_in, float *f_o
   }
   const void *cur = f_in;
   void *bufs[2] = {wa, wb};
+  if (ib->pipelined) {
+    // window 2+ cells inside the grid, walls without shared corners
+    for (int a = 0; a < 3; ++a)
+      VSX_REQUIRE(ib->origin[a] >= 2 && ib->origin[a] + ib->win[a] + 2 <= c.dims[a],
+                  "the pipelined sphere step needs the IB window 2+ cells inside the grid (axis %d)", a);
+    VSX_REQUIRE(!(p->n_walls > 0 && walls_need_edge<D3Q19>(*p)), "the pipelined sphere step needs walls without "
+                "shared corners");
+    VSX_REQUIRE(ib->halo, "the pipelined sphere step needs the hand-off buffer ib->halo");
+    PipeRes *r = nullptr;
+    if ((rc = pipe_res(&r))) return rc;
+    const int n_halo = (int)halo_blocks(c, d);
+    // flags restart with the run (a fill kernel, not a memset: the run may be captured into a graph)
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, d.flags, (int64_t)4, 0);
+    VSX_HIP(hipEventRecord(r->fork, s));
+    VSX_HIP(hipStreamWaitEvent(r->s, r->fork, 0));
+    for (int t = 0; t < n; ++t) {
+      rc = ib3d_phase(c, d, f_in, t == 0, ib->hist_h ? ib->hist_h + 3 * (int64_t)t : nullptr, r->s, ib->halo, t);
+      if (rc) return rc;
+      void *dst = bufs[t & 1];
+      switch (c.p.storage) {
+        case VSX_STORE_F32: bulk3d_pipe_any<Store<VSX_STORE_F32>>(c, d, cur, dst, ib->halo, t, n_halo, t == 0, t == n - 1, s); break;
+        case VSX_STORE_F16S: bulk3d_pipe_any<Store<VSX_STORE_F16S>>(c, d, cur, dst, ib->halo, t, n_halo, t == 0, t == n - 1, s); break;
+        case VSX_STORE_BF16S: bulk3d_pipe_any<Store<VSX_STORE_BF16S>>(c, d, cur, dst, ib->halo, t, n_halo, t == 0, t == n - 1, s); break;
+        default: bulk3d_pipe_any<Store<VSX_STORE_E3M12S>>(c, d, cur, dst, ib->halo, t, n_halo, t == 0, t == n - 1, s); break;
+      }
+      VSX_CHECK_LAUNCH();
+      cur = dst;
+    }
+    VSX_HIP(hipEventRecord(r->join, r->s));
+    VSX_HIP(hipStreamWaitEvent(s, r->join, 0));
+    const StepCtx cs = make_ctx<D3Q19>(*p, Window{nullptr, nullptr, {0, 0, 0}});
+    return step_any<D3Q19>(cs, cur, false, f_out, true, VSX_PHASE_STREAM, s);
+  }
   for (int t = 0; t < n; ++t) {
     void *dst = bufs[t & 1];
     rc = ib3d_phase(c, d, cur, t == 0, ib->hist_h ? ib->hist_h + 3 * (int64_t)t : nullptr, s);

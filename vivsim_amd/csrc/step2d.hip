@@ -54,6 +54,7 @@ int fused_full(const StepCtx &c, const VsxIB2D &ib, const void *src, void *dst, 
   return fused_mode_any<kModeFull>(c, ib, src, dst, step, s);
 }
 
+
 int viv_run(const VsxLbm *p, const VsxIB2D *ib, const float *f_in, float *f_out, void *wa, void *wb, int n,
             void *stream) {
   int rc = validate_lbm<D2Q9>(p);
@@ -77,6 +78,28 @@ int viv_run(const VsxLbm *p, const VsxIB2D *ib, const float *f_in, float *f_out,
   const bool push = fused && n > 1 && walls_allow_push(*p);
   const void *cur = f_in;
   void *bufs[2] = {wa, wb};
+  if (fused && ib->fused == 2) {
+    // pipelined: IB kernels on the second stream, bulk passes on the caller's
+    VSX_REQUIRE(ib->halo, "the pipelined VIV step needs the hand-off buffer ib->halo");
+    PipeRes *r = nullptr;
+    if ((rc = pipe_res(&r))) return rc;
+    VSX_HIP(hipEventRecord(r->fork, s));
+    VSX_HIP(hipStreamWaitEvent(r->s, r->fork, 0));
+    for (int t = 0; t < n; ++t) {
+      if ((rc = launch_ib_pipe(c, *ib, f_in, t == 0, t, r->s))) return rc;
+      void *dst = (push && t == n - 1) ? f_out : bufs[t & 1];
+      if (t == 0) rc = fused_first(c, *ib, cur, dst, t, s);
+      else if (push && t == n - 1) rc = fused_last(c, *ib, cur, dst, t, s);
+      else rc = fused_full(c, *ib, cur, dst, t, s);
+      if (rc) return rc;
+      cur = dst;
+    }
+    VSX_HIP(hipEventRecord(r->join, r->s));
+    VSX_HIP(hipStreamWaitEvent(s, r->join, 0));
+    if (push) return walls_in_place<D2Q9>(c, f_out, s);
+    const StepCtx cs = make_ctx<D2Q9>(*p, Window{nullptr, nullptr, {0, 0, 0}});
+    return step_any<D2Q9>(cs, cur, false, f_out, true, VSX_PHASE_STREAM, s);
+  }
   for (int t = 0; t < n; ++t) {
     void *dst = bufs[t & 1];
     if (!fused) {

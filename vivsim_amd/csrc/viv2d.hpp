@@ -979,13 +979,19 @@ __device__ __forceinline__ void spread_rows(float *grid, int base, int S, const 
 //   body force   h = sum_m (-F_m) + a * area in FP64, dyn.py:126-136
 //   structure    Newmark-beta in FP64 on thread 0, dyn.py:5-51
 // Writes the force window to ib.g_win (st_agent) and the new body state to body_out.
-template <class SRC, bool PULL, int UW, int TAG>
+// Where the IB workgroup takes the window velocity from.
+constexpr int kWinPull = 0;    // pull from the post-collision state src (storage SRC) with walls
+constexpr int kWinLocal = 1;   // read src in place (a post-streaming state)
+constexpr int kWinHalo = 2;    // pull from the FP32 hand-off buffer `halo` [9][S + 4][S + 4] whose origin
+                               // is the previous step's window origin - 2 (ib.win_origin[0..1])
+
+template <class SRC, int WSRC, int UW, int TAG>
 __device__ __forceinline__ void ib_atomic(const StepCtx &c, const VsxIB2D &ib, const typename SRC::T *__restrict__ src,
-                                          int hidx, int gstep, int32_t *go_flag, const double *body_in,
-                                          double *body_out, float *lds) {
+                                          const float *halo, int hidx, int gstep, int32_t *go_flag,
+                                          const double *body_in, double *body_out, float *lds) {
   const int S = ib.win, SS = S * S, M = ib.n_markers;
   __shared__ double s_body[6];
-  __shared__ int s_org[4];
+  __shared__ int s_org[6];
   __shared__ double s_red[2][kIbThreads / 64];
   const int tid = threadIdx.x, nt = blockDim.x;
   const IbLane l = ib_lane(M);
@@ -998,6 +1004,16 @@ __device__ __forceinline__ void ib_atomic(const StepCtx &c, const VsxIB2D &ib, c
     window_origin(c, ib, s_body, o);
 #pragma unroll
     for (int k = 0; k < 4; ++k) s_org[k] = o[k];
+    if constexpr (WSRC == kWinHalo) {
+      // the hand-off buffer covers the previous window +- 2 cells: this window
+      // must lie within 1 cell of it and 2 cells inside the grid
+      const int ex = ib.win_origin[0] - 2, ey = ib.win_origin[1] - 2;
+      const bool ok = o[2] - ex >= 1 && o[2] - ex <= 3 && o[3] - ey >= 1 && o[3] - ey <= 3 && o[2] >= 2 &&
+                      o[3] >= 2 && o[2] + S + 2 <= c.dims[0] && o[3] + S + 2 <= c.dims[1];
+      if (!ok) __hip_atomic_store(&ib.win_origin[3], 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s_org[4] = ex;
+      s_org[5] = ey;
+    }
     ib.win_origin[0] = o[2];
     ib.win_origin[1] = o[3];
   }
@@ -1006,25 +1022,40 @@ __device__ __forceinline__ void ib_atomic(const StepCtx &c, const VsxIB2D &ib, c
 
   // window velocity (float2 per cell over the start of the accumulation grid)
   float2 *ugrid = reinterpret_cast<float2 *>(lds);
-  for (int base = 0; base < SS; base += UW * nt) {
-    float fu[UW][9];
-    bool fu_on[UW];
-    window_pull<SRC, PULL, UW>(c, src, base, S, xc, yc, fu, fu_on);
-    window_store<UW>(ugrid, base, fu, fu_on);
-  }
-  if (PULL && c.p.n_walls > 0) {
-    // window cells on a domain wall (rare: only when the window touches a
-    // wall), inline: an out-of-line call would make the caller save live
-    // registers.  Fused steps have no wall corners (every wall cell is of a
-    // single-wall class).
+  if constexpr (WSRC == kWinHalo) {
+    const int E = S + 4, EE = E * E, dx = xc - s_org[4], dy = yc - s_org[5];
+    const bool ok = dx >= 1 && dx <= 3 && dy >= 1 && dy <= 3;
     for (int k = tid; k < SS; k += nt) {
-      const int co[3] = {xc + k / S, yc + k % S, 0};
-      const int cls = wall_class<D2Q9>(c, co);
-      if (cls >= 0) {
-        float f[9], rho, u[3];
-        single_wall_state<D2Q9, SRC>(c, src, co, cls, f);
-        macroscopic<D2Q9>(f, rho, u);
-        ugrid[k] = make_float2(u[0], u[1]);
+      float f[9], rho, u[3];
+      const int i = k / S + dx, j = k % S + dy;
+#pragma unroll
+      for (int q = 0; q < 9; ++q)
+        f[q] = ok ? ld_agent(&halo[q * EE + (i - D2Q9::cx[q]) * E + (j - D2Q9::cy[q])]) : D2Q9::w[q];
+      macroscopic<D2Q9>(f, rho, u);
+      ugrid[k] = make_float2(u[0], u[1]);
+    }
+  } else {
+    constexpr bool PULL = WSRC == kWinPull;
+    for (int base = 0; base < SS; base += UW * nt) {
+      float fu[UW][9];
+      bool fu_on[UW];
+      window_pull<SRC, PULL, UW>(c, src, base, S, xc, yc, fu, fu_on);
+      window_store<UW>(ugrid, base, fu, fu_on);
+    }
+    if (PULL && c.p.n_walls > 0) {
+      // window cells on a domain wall (rare: only when the window touches a
+      // wall), inline: an out-of-line call would make the caller save live
+      // registers.  Fused steps have no wall corners (every wall cell is of a
+      // single-wall class).
+      for (int k = tid; k < SS; k += nt) {
+        const int co[3] = {xc + k / S, yc + k % S, 0};
+        const int cls = wall_class<D2Q9>(c, co);
+        if (cls >= 0) {
+          float f[9], rho, u[3];
+          single_wall_state<D2Q9, SRC>(c, src, co, cls, f);
+          macroscopic<D2Q9>(f, rho, u);
+          ugrid[k] = make_float2(u[0], u[1]);
+        }
       }
     }
   }
@@ -1124,7 +1155,7 @@ __device__ __forceinline__ void ib_atomic(const StepCtx &c, const VsxIB2D &ib, c
       out[2 + k] = on ? c1 * an + v1 : 0.0;
       out[4 + k] = on ? c2 * an + v2 : 0.0;
     }
-    for (int k = 0; k < 6; ++k) body_out[k] = out[k];
+    for (int k = 0; k < 6; ++k) __hip_atomic_store(&body_out[k], out[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (ib.hist_d && hidx >= 0) {
       ib.hist_d[2 * hidx] = out[4];
       ib.hist_d[2 * hidx + 1] = out[5];
@@ -1178,6 +1209,22 @@ __device__ __forceinline__ void count_window_tile(const VsxIB2D &ib, int gstep, 
         __hip_atomic_fetch_add(&ib.win_origin[4 + (gstep & 1)], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (before == n_wtiles - 1)
       __hip_atomic_store(&ib.win_origin[6], gstep + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// Pipelined form: once every thread's hand-off stores have completed, count
+// the tile in ib.win_origin[4]; the last of the n hand-off tiles of step gstep
+// resets the counter (the next step's tiles run in a later launch of the same
+// stream) and raises ib.win_origin[6] to gstep + 1.
+__device__ __forceinline__ void count_halo_tile(const VsxIB2D &ib, int gstep, int n) {
+  wait_mem();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int before = __hip_atomic_fetch_add(&ib.win_origin[4], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (before == n - 1) {
+      __hip_atomic_store(&ib.win_origin[4], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&ib.win_origin[6], gstep + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
   }
 }
 
@@ -1239,6 +1286,7 @@ inline int validate_ib2d(const VsxLbm *p, const VsxIB2D *ib) {
   VSX_REQUIRE(ib->kernel >= VSX_KERNEL_PESKIN_2PT && ib->kernel <= VSX_KERNEL_COSINE_4PT, "unknown kernel %d", ib->kernel);
   VSX_REQUIRE(ib->n_iter >= 0, "negative n_iter");
   VSX_REQUIRE(ib->step0 >= 0, "negative step0");
+  VSX_REQUIRE(ib->fused >= 0 && ib->fused <= 2, "fused must be 0, 1 or 2, got %d", ib->fused);
   VSX_REQUIRE(ib->marker_x && ib->marker_y && ib->body && ib->g_win && ib->win_origin, "null IB buffer");
   VSX_REQUIRE(ib->mass + ib->gamma * ib->dt * ib->damping + ib->beta * ib->dt * ib->dt * ib->stiffness != 0.0,
               "singular Newmark system");

@@ -13,6 +13,16 @@
 //   window tiles  pull their cells, wait (bounded) for the ready flag, read the
 //                 force window with ld_agent, collide
 //   bulk tiles    wait (bounded) for ib.win_origin[6], then stream and collide
+// Pipelined form (ib.fused == 2): there is no IB block; the IB of step t + 1
+// runs as its own kernel (k_ib_pipe) on a second stream while the bulk pass of
+// step t streams.  The tiles of step t that cover the window +- 2 cells come
+// first; they write their new populations (as read back from storage) into
+// the FP32 hand-off buffer ib.halo (two halves, by step parity) with
+// st_agent, count themselves in
+// ib.win_origin[4], and the last one raises ib.win_origin[6]; the IB kernel of
+// step t + 1 waits for that flag and pulls the window from the buffer.  The
+// force-window tiles wait for the IB kernel's ready flag as in the one-launch
+// form; no other tile waits.
 // Modes (the post-collision state is what the work buffers hold):
 //   kModeFull   src = post-collision state of the previous step (pull + walls)
 //   kModeFirst  src = the caller's post-streaming FP32 field, read in place (the
@@ -125,27 +135,42 @@ k_viv_fused(StepCtx c_arg, VsxIB2D ib, const typename SRC::T *__restrict__ src, 
   extern __shared__ float lds[];
   const int gstep = ib.step0 + step, slot = gstep & 1;
   const double *body_in = ib.body + 6 * slot;
+  const bool pipe = ib.fused == 2;
   __shared__ int s_org[2];
   if (threadIdx.x == 0) {
+    double b[6];
+    b[4] = __hip_atomic_load(const_cast<double *>(body_in + 4), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    b[5] = __hip_atomic_load(const_cast<double *>(body_in + 5), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     int o[4];
-    window_origin(c, ib, body_in, o);
+    window_origin(c, ib, b, o);
     s_org[0] = o[2];
     s_org[1] = o[3];
   }
   __syncthreads();
   const int org[3] = {s_org[0], s_org[1], 0};
+  // first tiles: the force window's (one-launch form) or the window +- 2 cells (pipelined form)
   int wx0, wx1, wy0, wy1;
-  window_tiles(T, org[0], org[1], ib.win, wx0, wx1, wy0, wy1);
-  if (blockIdx.x == 0) {
-    ib_atomic<SRC, MODE != kModeFirst, 2, 1>(c, ib, src, step, gstep, &ib.win_origin[6], body_in,
-                                             ib.body + 6 * (slot ^ 1), lds);
+  if (pipe) {
+    // clamped to the grid (a window closer than 2 cells to the edge is an error the IB kernel reports)
+    wx0 = max(org[0] - 2, 0) / T.TX;
+    wx1 = min(org[0] + ib.win + 1, c.dims[0] - 1) / T.TX;
+    wy0 = max(org[1] - 2, 0) / T.TY;
+    wy1 = min(org[1] + ib.win + 1, c.dims[1] - 1) / T.TY;
+  } else {
+    window_tiles(T, org[0], org[1], ib.win, wx0, wx1, wy0, wy1);
+  }
+  if (!pipe && blockIdx.x == 0) {
+    ib_atomic<SRC, MODE == kModeFirst ? kWinLocal : kWinPull, 2, 1>(c, ib, src, nullptr, step, gstep,
+                                                                     &ib.win_origin[6], body_in,
+                                                                     ib.body + 6 * (slot ^ 1), lds);
     publish_ready(ib, gstep + 1);
     return;
   }
+  const int bid = pipe ? (int)blockIdx.x : (int)blockIdx.x - 1;   // tile-block index
   const int n_win = (wx1 - wx0 + 1) * (wy1 - wy0 + 1);
   if constexpr (PAIR) {
-    if ((int)blockIdx.x - 1 >= n_win) {
-      const int q = n_win + 2 * ((int)blockIdx.x - 1 - n_win), n_tiles = T.ntx * T.nty;
+    if (bid >= n_win) {
+      const int q = n_win + 2 * (bid - n_win), n_tiles = T.ntx * T.nty;
       if (q >= n_tiles) return;
       int ta[2], tb[2];
       bool w_;
@@ -155,8 +180,10 @@ k_viv_fused(StepCtx c_arg, VsxIB2D ib, const typename SRC::T *__restrict__ src, 
       const int a[3] = {ta[0] * T.TX + (int)threadIdx.x / T.TY, ta[1] * T.TY + (int)threadIdx.x % T.TY, 0};
       const int b[3] = {has_b ? tb[0] * T.TX + (int)threadIdx.x / T.TY : c.dims[0],
                         has_b ? tb[1] * T.TY + (int)threadIdx.x % T.TY : c.dims[1], 0};
-      if (threadIdx.x == 0) spin_until(ib, &ib.win_origin[6], gstep + 1);
-      __syncthreads();
+      if (!pipe) {
+        if (threadIdx.x == 0) spin_until(ib, &ib.win_origin[6], gstep + 1);
+        __syncthreads();
+      }
       const bool va = a[0] < c.dims[0] && a[1] < c.dims[1], vb = b[0] < c.dims[0] && b[1] < c.dims[1];
       const bool wa = va && c.p.n_walls > 0 && wall_class<D2Q9>(c, a) >= 0;
       const bool wb = vb && c.p.n_walls > 0 && wall_class<D2Q9>(c, b) >= 0;
@@ -172,10 +199,36 @@ k_viv_fused(StepCtx c_arg, VsxIB2D ib, const typename SRC::T *__restrict__ src, 
   }
   int tx, ty;
   bool in_win;
-  tile_of_block(blockIdx.x - 1, T, wx0, wx1, wy0, wy1, tx, ty, in_win);
+  tile_of_block(bid, T, wx0, wx1, wy0, wy1, tx, ty, in_win);
   const int co[3] = {tx * T.TX + (int)threadIdx.x / T.TY, ty * T.TY + (int)threadIdx.x % T.TY, 0};
   const bool valid = co[0] < c.dims[0] && co[1] < c.dims[1];
   float f[9];
+  if (pipe && in_win) {
+    // tile around the window (pipelined form): collide (after the force window
+    // when the tile intersects it), store, hand the cells within 2 of the window
+    // to the next step's IB kernel
+    const int S = ib.win;
+    const bool force = tx * T.TX < org[0] + S && (tx + 1) * T.TX > org[0] && ty * T.TY < org[1] + S &&
+                       (ty + 1) * T.TY > org[1];
+    if (valid) tile_cell_state<SRC, MODE>(c, src, co, f);
+    if (force) wait_ready(ib, gstep + 1);
+    if (valid) {
+      finish<D2Q9, COLL>(c, cell_of<D2Q9>(c, co[0], co[1], 0), co, org, force, f);
+      tile_store<DST, MODE>(c, dst, co, f);
+    }
+    if constexpr (MODE != kModeLast) {
+      // hand-off buffer of this step's parity (the next step's IB reads it while the
+      // next bulk pass writes the other one)
+      const int E = S + 4, EE = E * E, a = co[0] - (org[0] - 2), b = co[1] - (org[1] - 2);
+      float *h = ib.halo + (size_t)(gstep & 1) * 9 * EE;
+      if (valid && a >= 0 && a < E && b >= 0 && b < E) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) st_agent(&h[q * EE + a * E + b], DST::rt(f[q], c.shift[q]));
+      }
+      count_halo_tile(ib, gstep, n_win);
+    }
+    return;
+  }
   if (in_win) {
     // window tile: post-streaming state -> wait for the IB workgroup's force window -> collide
     if (valid) tile_cell_state<SRC, MODE>(c, src, co, f);
@@ -185,9 +238,11 @@ k_viv_fused(StepCtx c_arg, VsxIB2D ib, const typename SRC::T *__restrict__ src, 
     tile_store<DST, MODE>(c, dst, co, f);
     return;
   }
-  // bulk tile: wait (bounded) until the IB workgroup has pulled the window
-  if (threadIdx.x == 0) spin_until(ib, &ib.win_origin[6], gstep + 1);
-  __syncthreads();
+  // bulk tile (one-launch form): wait (bounded) until the IB workgroup has pulled the window
+  if (!pipe) {
+    if (threadIdx.x == 0) spin_until(ib, &ib.win_origin[6], gstep + 1);
+    __syncthreads();
+  }
   if (!valid) return;
   if constexpr (MODE != kModeFirst) {
     const int cls = c.p.n_walls > 0 ? wall_class<D2Q9>(c, co) : -1;
@@ -204,6 +259,28 @@ k_viv_fused(StepCtx c_arg, VsxIB2D ib, const typename SRC::T *__restrict__ src, 
   }
   finish<D2Q9, COLL>(c, cell_of<D2Q9>(c, co[0], co[1], 0), co, org, false, f);
   tile_store<DST, MODE>(c, dst, co, f);
+}
+
+// IB kernel of the pipelined form: step gstep's IB, pulling the window from the
+// hand-off buffer (written by the previous step's bulk pass, WSRC = kWinHalo;
+// it waits for that pass's flag) or, for the first step of a run, reading the
+// caller's post-streaming FP32 field in place (kWinLocal).  One block of 512
+// (M <= 512) or 1024 threads; its LDS and registers do not constrain the bulk
+// kernel's occupancy.
+template <int WSRC>
+__global__ __launch_bounds__(kIbThreads) void k_ib_pipe(StepCtx c_arg, VsxIB2D ib, const float *__restrict__ src,
+                                                       int step) {
+  const StepCtx &c = kernel_ctx();
+  extern __shared__ float lds[];
+  const int gstep = ib.step0 + step, slot = gstep & 1;
+  if constexpr (WSRC == kWinHalo) {
+    if (threadIdx.x == 0) spin_until(ib, &ib.win_origin[6], gstep);
+    __syncthreads();
+  }
+  const float *halo = ib.halo + (size_t)((gstep - 1) & 1) * 9 * (ib.win + 4) * (ib.win + 4);
+  ib_atomic<Store<VSX_STORE_F32>, WSRC, 2, 2>(c, ib, src, halo, step, gstep, nullptr, ib.body + 6 * slot,
+                                               ib.body + 6 * (slot ^ 1), lds);
+  publish_ready(ib, gstep + 1);
 }
 
 // One wall applied in place to a post-streaming FP32 field (the reference's
@@ -280,17 +357,20 @@ constexpr int kPairMinTiles = 4096;
 template <class SRC, class DST, int COLL, bool PAIR, int MODE>
 int launch_fused_p(const StepCtx &c, const VsxIB2D &ib, const void *src, void *dst, int step, hipStream_t s,
                    const Tiles &T) {
-  const size_t lds = 4 * (size_t)ib_atomic_lds_words(ib.win);
+  const bool pipe = ib.fused == 2;
+  const size_t lds = pipe ? 0 : 4 * (size_t)ib_atomic_lds_words(ib.win);
   const int n_tiles = T.ntx * T.nty;
   int grid = n_tiles;
   if (PAIR) {
-    const int nw_max = ((ib.win + T.TX - 1) / T.TX + 1) * ((ib.win + T.TY - 1) / T.TY + 1);
+    const int side = pipe ? ib.win + 4 : ib.win;
+    const int nw_max = ((side + T.TX - 1) / T.TX + 1) * ((side + T.TY - 1) / T.TY + 1);
     grid = (n_tiles + nw_max + 1) / 2;
     if (grid > n_tiles) grid = n_tiles;
   }
   timer_begin(s);
-  hipLaunchKernelGGL((k_viv_fused<SRC, DST, COLL, PAIR, MODE>), dim3(1 + grid), dim3(kTileCells), lds, s, c, ib,
-                     static_cast<const typename SRC::T *>(src), static_cast<typename DST::T *>(dst), step, T);
+  hipLaunchKernelGGL((k_viv_fused<SRC, DST, COLL, PAIR, MODE>), dim3((pipe ? 0 : 1) + grid), dim3(kTileCells), lds,
+                     s, c, ib, static_cast<const typename SRC::T *>(src), static_cast<typename DST::T *>(dst), step,
+                     T);
   timer_end(s);
   VSX_CHECK_LAUNCH();
   return VSX_OK;
@@ -327,6 +407,17 @@ int fused_mode_any(const StepCtx &c, const VsxIB2D &ib, const void *src, void *d
     case VSX_STORE_BF16S: return launch_fused_mode<Store<VSX_STORE_BF16S>, MODE>(c, ib, src, dst, step, s);
     default: return launch_fused_mode<Store<VSX_STORE_E3M12S>, MODE>(c, ib, src, dst, step, s);
   }
+}
+
+// IB kernel of step `step` of the pipelined form (first: read the caller's field src in place)
+inline int launch_ib_pipe(const StepCtx &c, const VsxIB2D &ib, const float *src, bool first, int step,
+                          hipStream_t s) {
+  const size_t lds = 4 * (size_t)ib_atomic_lds_words(ib.win);
+  const int nt = ib.n_markers <= kIbThreads / 2 ? kIbThreads / 2 : kIbThreads;
+  if (first) hipLaunchKernelGGL((k_ib_pipe<kWinLocal>), dim3(1), dim3(nt), lds, s, c, ib, src, step);
+  else hipLaunchKernelGGL((k_ib_pipe<kWinHalo>), dim3(1), dim3(nt), lds, s, c, ib, src, step);
+  VSX_CHECK_LAUNCH();
+  return VSX_OK;
 }
 
 // defined in the translation unit that instantiates each mode

@@ -225,7 +225,12 @@ class _TimeoutWatch:
             self.event.synchronize()
         elif not self.event.query():
             return
-        bad = [k for k in range(len(self.flags)) if int(self.host[k]) != 0]
+        codes = [int(self.host[k]) for k in range(len(self.flags))]
+        if any(c == 2 for c in codes):
+            bad = [k for k, c in enumerate(codes) if c == 2]
+            raise RuntimeError(f"pipelined VIV step, case(s) {bad}: the IB window came closer than 2 cells to the "
+                               "grid edge or moved by more than a cell in a step; run with pipelined=False")
+        bad = [k for k, c in enumerate(codes) if c != 0]
         if bad:
             raise RuntimeError(f"fused VIV step: a tile of case(s) {bad} timed out waiting for the IB workgroup "
                                "(the force window of that step was not applied)")
@@ -240,6 +245,8 @@ class _Body2D:
         self.marker_y = torch.as_tensor(my, dtype=torch.float64, device=dev)
         S = case.ib_size
         self.g_win = torch.zeros(2 * S * S, dtype=torch.float32, device=dev)
+        # hand-off buffer of the pipelined step: two halves of [9][S + 4][S + 4]
+        self.halo = torch.zeros(2 * 9 * (S + 4) ** 2, dtype=torch.float32, device=dev)
         # origin x, y of the last step's window, ready flag, spin-timeout flag, window-tile counters
         self.win_origin = torch.zeros(8, dtype=torch.int32, device=dev)
         # two slots of [a0 a1 v0 v1 d0 d1]: slot (t & 1) holds the state before step t
@@ -259,6 +266,7 @@ class _Body2D:
         ib.body = self.body.data_ptr()
         ib.g_win = self.g_win.data_ptr()
         ib.win_origin = self.win_origin.data_ptr()
+        ib.halo = self.halo.data_ptr()
         self.ib = ib
 
     def set(self, a, v, d, slot=0):
@@ -269,9 +277,17 @@ class _Body2D:
 
 class VIV2D:
     """Device state and fused run of a ``VIVCase``.  16-bit storages keep the
-    populations relative to the equilibrium of the inflow (U0, 0)."""
+    populations relative to the equilibrium of the inflow (U0, 0).
 
-    def __init__(self, case: VIVCase, storage="f32", fused=True):
+    fused=False: separate IB and bulk launches per step.  fused=True with
+    pipelined=True: the IB of step t + 1 runs as its own kernel on a second
+    stream while the bulk pass of step t streams (the IB window must stay at
+    least 2 cells inside the grid; a run that breaks this raises on the next
+    call); pipelined=False: one launch per step, the IB workgroup beside the
+    bulk tiles (any window position).  pipelined=None picks the pipelined form
+    when the initial window is at least 3 cells inside the grid."""
+
+    def __init__(self, case: VIVCase, storage="f32", fused=True, pipelined=None):
         self.case = case
         dev = L.device()
         self.lbm = LBM((case.nx, case.ny), case.omega, collision=case.collision, walls=case.walls(),
@@ -280,7 +296,11 @@ class VIV2D:
         self.body, self.win_origin, self.g_win = self._b.body, self._b.win_origin, self._b.g_win
         self.t_total = 0
         self.ib = self._b.ib
-        self.ib.fused = int(bool(fused))
+        if pipelined is None:
+            S = case.ib_size
+            pipelined = min(case.ib_x0, case.ib_y0) >= 3 and case.ib_x0 + S + 3 <= case.nx and \
+                case.ib_y0 + S + 3 <= case.ny
+        self.ib.fused = (2 if pipelined else 1) if fused else 0
         self._watch = _TimeoutWatch([self.win_origin[3:4]])
 
     def set_body(self, a=(0.0, 0.0), v=(0.0, 0.0), d=(0.0, 0.0)):
@@ -301,12 +321,18 @@ class VIV2D:
         self._watch.poll(wait=True)
 
     def run(self, f, n_steps, out=None, history=True):
-        """Advance n_steps; returns (f_out, d_hist [n,2], h_hist [n,2]) (histories FP64 on device)."""
+        """Advance n_steps; returns (f_out, d_hist [n,2], h_hist [n,2]) (histories FP64 on device).
+        ``history`` may also be a pair of caller-owned FP64 [>= n_steps, 2] device
+        tensors (d, h) to fill, which spares the allocations in a loop of runs."""
         self._watch.poll()
         f = f32(f)
         out = torch.empty_like(f) if out is None else out
         hd = hh = None
-        if history and n_steps > 0:
+        if isinstance(history, tuple):
+            hd, hh = history
+            require(hd.dtype == hh.dtype == torch.float64 and hd.shape[0] >= n_steps and hh.shape[0] >= n_steps
+                    and hd.is_contiguous() and hh.is_contiguous(), "history buffers must be contiguous FP64 [n, 2]")
+        elif history and n_steps > 0:
             hd = torch.empty((n_steps, 2), dtype=torch.float64, device=f.device)
             hh = torch.empty((n_steps, 2), dtype=torch.float64, device=f.device)
         self.ib.hist_d = None if hd is None else hd.data_ptr()
@@ -394,9 +420,11 @@ class SphereCase:
 
 
 class Sphere3D:
-    """Device state and fused run of a ``SphereCase`` (``vsx_sphere3d_run``)."""
+    """Device state and fused run of a ``SphereCase`` (``vsx_sphere3d_run``).
+    pipelined (None: when the window is 2+ cells inside the grid): the IB of
+    step t + 1 runs on a second stream while the bulk pass of step t streams."""
 
-    def __init__(self, case: SphereCase, storage="f32"):
+    def __init__(self, case: SphereCase, storage="f32", pipelined=None):
         from . import ib3d
         self.case = case
         dev = L.device()
@@ -428,6 +456,11 @@ class Sphere3D:
         ib.ds_arr = None
         ib.ds = case.ds
         ib.g_win, ib.work = self.g_win.data_ptr(), self.work.data_ptr()
+        if pipelined is None:
+            pipelined = all(o >= 2 and o + S + 2 <= n for o, n in zip(org, (case.nx, case.ny, case.nz)))
+        self.halo = torch.zeros(2 * 19 * (S + 2) ** 3 if pipelined else 1, dtype=torch.float32, device=dev)
+        ib.pipelined = int(bool(pipelined))
+        ib.halo = self.halo.data_ptr()
         self.ib = ib
 
     def initial_f(self):
@@ -449,7 +482,19 @@ class Sphere3D:
         wa, wb = self.lbm.work()
         L.call("vsx_sphere3d_run", ctypes.byref(self.lbm.p), ctypes.byref(self.ib), L.ptr(f), L.ptr(out), L.ptr(wa),
                L.ptr(wb), int(n_steps))
+        self._check_flags()
         return out, hh
+
+    def _check_flags(self):
+        """Raise if a bounded wait of an earlier pipelined run ran out (no host sync: the
+        flag word is copied asynchronously and read once its copy has landed)."""
+        if not self.ib.pipelined:
+            return
+        if getattr(self, "_watch", None) is None:
+            flag = self.work[7:8].view(torch.int32)
+            self._watch = _TimeoutWatch([flag])
+        self._watch.poll()
+        self._watch.after_run()
 
 
 # ====================================================================== batched VIV sweep
@@ -656,7 +701,7 @@ class VIVAdjoint:
         lb, body = self._setup(delta, stiffness, u_in)
         hd = torch.empty((self.T, 2), dtype=torch.float64, device=L.device())
         body.ib.hist_d, body.ib.hist_h = hd.data_ptr(), None
-        body.ib.fused = 1
+        body.ib.fused = 2
         body.ib.step0 = 0
         wa, wb = lb.work()
         out = torch.empty_like(self.f0)
