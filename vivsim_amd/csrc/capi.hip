@@ -68,9 +68,26 @@ int pipe_res(PipeRes **out) {
   std::lock_guard<std::mutex> lock(g_pipe_mu);
   PipeRes &r = g_pipe[dev];
   if (!r.s) {
-    VSX_HIP(hipStreamCreateWithFlags(&r.s, hipStreamNonBlocking));
+    int n_cu = 0;
+    VSX_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+    std::vector<uint32_t> ib_mask((n_cu + 31) / 32, 0u), bulk_mask((n_cu + 31) / 32, 0u);
+    for (int k = 0; k < n_cu; ++k) {
+      if (k == n_cu - 1) ib_mask[k / 32] |= 1u << (k % 32);
+      else bulk_mask[k / 32] |= 1u << (k % 32);
+    }
+    r.masked = n_cu > 1 &&
+               hipExtStreamCreateWithCUMask(&r.s, (uint32_t)ib_mask.size(), ib_mask.data()) == hipSuccess &&
+               hipExtStreamCreateWithCUMask(&r.b, (uint32_t)bulk_mask.size(), bulk_mask.data()) == hipSuccess;
+    if (!r.masked) {
+      (void)hipGetLastError();
+      if (r.s) (void)hipStreamDestroy(r.s);
+      if (r.b) (void)hipStreamDestroy(r.b);
+      VSX_HIP(hipStreamCreateWithFlags(&r.s, hipStreamNonBlocking));
+      VSX_HIP(hipStreamCreateWithFlags(&r.b, hipStreamNonBlocking));
+    }
     VSX_HIP(hipEventCreateWithFlags(&r.fork, hipEventDisableTiming));
-    VSX_HIP(hipEventCreateWithFlags(&r.join, hipEventDisableTiming));
+    VSX_HIP(hipEventCreateWithFlags(&r.join_s, hipEventDisableTiming));
+    VSX_HIP(hipEventCreateWithFlags(&r.join_b, hipEventDisableTiming));
   }
   *out = &r;
   return VSX_OK;

@@ -19,13 +19,18 @@ int fail(int code, const char *fmt, ...);
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
-// Second stream of the pipelined steps (default priority, non-blocking) and the
-// two events that fork it from and join it to the caller's stream: one set per
-// device, created on first use and kept for the process.  Calls that use it
-// from several host threads at once on one device are not supported.
+// Streams of the pipelined steps (default priority, non-blocking): `s` runs the
+// IB kernels on a compute-unit mask of one CU, `b` the bulk passes on all the
+// other CUs, so the latency-bound IB never shares its CU with streaming
+// blocks and never waits for one to drain; events fork both from the caller's
+// stream and join them back.  One set per device, created on first use and
+// kept for the process (if the runtime refuses CU masks, both are plain
+// streams).  Calls that use them from several host threads at once on one
+// device are not supported.
 struct PipeRes {
-  hipStream_t s = nullptr;
-  hipEvent_t fork = nullptr, join = nullptr;
+  hipStream_t s = nullptr, b = nullptr;
+  hipEvent_t fork = nullptr, join_s = nullptr, join_b = nullptr;
+  bool masked = false;
 };
 int pipe_res(PipeRes **out);
 

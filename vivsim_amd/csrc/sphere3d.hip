@@ -513,21 +513,24 @@ int sphere_run(const VsxLbm *p, const VsxIB3D *ib, const float *f_in, float *f_o
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, d.flags, (int64_t)4, 0);
     VSX_HIP(hipEventRecord(r->fork, s));
     VSX_HIP(hipStreamWaitEvent(r->s, r->fork, 0));
+    VSX_HIP(hipStreamWaitEvent(r->b, r->fork, 0));
     for (int t = 0; t < n; ++t) {
       rc = ib3d_phase(c, d, f_in, t == 0, ib->hist_h ? ib->hist_h + 3 * (int64_t)t : nullptr, r->s, ib->halo, t);
       if (rc) return rc;
       void *dst = bufs[t & 1];
       switch (c.p.storage) {
-        case VSX_STORE_F32: bulk3d_pipe_any<Store<VSX_STORE_F32>>(c, d, cur, dst, ib->halo, t, n_halo, t == 0, t == n - 1, s); break;
-        case VSX_STORE_F16S: bulk3d_pipe_any<Store<VSX_STORE_F16S>>(c, d, cur, dst, ib->halo, t, n_halo, t == 0, t == n - 1, s); break;
-        case VSX_STORE_BF16S: bulk3d_pipe_any<Store<VSX_STORE_BF16S>>(c, d, cur, dst, ib->halo, t, n_halo, t == 0, t == n - 1, s); break;
-        default: bulk3d_pipe_any<Store<VSX_STORE_E3M12S>>(c, d, cur, dst, ib->halo, t, n_halo, t == 0, t == n - 1, s); break;
+        case VSX_STORE_F32: bulk3d_pipe_any<Store<VSX_STORE_F32>>(c, d, cur, dst, ib->halo, t, n_halo, t == 0, t == n - 1, r->b); break;
+        case VSX_STORE_F16S: bulk3d_pipe_any<Store<VSX_STORE_F16S>>(c, d, cur, dst, ib->halo, t, n_halo, t == 0, t == n - 1, r->b); break;
+        case VSX_STORE_BF16S: bulk3d_pipe_any<Store<VSX_STORE_BF16S>>(c, d, cur, dst, ib->halo, t, n_halo, t == 0, t == n - 1, r->b); break;
+        default: bulk3d_pipe_any<Store<VSX_STORE_E3M12S>>(c, d, cur, dst, ib->halo, t, n_halo, t == 0, t == n - 1, r->b); break;
       }
       VSX_CHECK_LAUNCH();
       cur = dst;
     }
-    VSX_HIP(hipEventRecord(r->join, r->s));
-    VSX_HIP(hipStreamWaitEvent(s, r->join, 0));
+    VSX_HIP(hipEventRecord(r->join_s, r->s));
+    VSX_HIP(hipEventRecord(r->join_b, r->b));
+    VSX_HIP(hipStreamWaitEvent(s, r->join_s, 0));
+    VSX_HIP(hipStreamWaitEvent(s, r->join_b, 0));
     const StepCtx cs = make_ctx<D3Q19>(*p, Window{nullptr, nullptr, {0, 0, 0}});
     return step_any<D3Q19>(cs, cur, false, f_out, true, VSX_PHASE_STREAM, s);
   }

@@ -85,17 +85,20 @@ int viv_run(const VsxLbm *p, const VsxIB2D *ib, const float *f_in, float *f_out,
     if ((rc = pipe_res(&r))) return rc;
     VSX_HIP(hipEventRecord(r->fork, s));
     VSX_HIP(hipStreamWaitEvent(r->s, r->fork, 0));
+    VSX_HIP(hipStreamWaitEvent(r->b, r->fork, 0));
     for (int t = 0; t < n; ++t) {
       if ((rc = launch_ib_pipe(c, *ib, f_in, t == 0, t, r->s))) return rc;
       void *dst = (push && t == n - 1) ? f_out : bufs[t & 1];
-      if (t == 0) rc = fused_first(c, *ib, cur, dst, t, s);
-      else if (push && t == n - 1) rc = fused_last(c, *ib, cur, dst, t, s);
-      else rc = fused_full(c, *ib, cur, dst, t, s);
+      if (t == 0) rc = fused_first(c, *ib, cur, dst, t, r->b);
+      else if (push && t == n - 1) rc = fused_last(c, *ib, cur, dst, t, r->b);
+      else rc = fused_full(c, *ib, cur, dst, t, r->b);
       if (rc) return rc;
       cur = dst;
     }
-    VSX_HIP(hipEventRecord(r->join, r->s));
-    VSX_HIP(hipStreamWaitEvent(s, r->join, 0));
+    VSX_HIP(hipEventRecord(r->join_s, r->s));
+    VSX_HIP(hipEventRecord(r->join_b, r->b));
+    VSX_HIP(hipStreamWaitEvent(s, r->join_s, 0));
+    VSX_HIP(hipStreamWaitEvent(s, r->join_b, 0));
     if (push) return walls_in_place<D2Q9>(c, f_out, s);
     const StepCtx cs = make_ctx<D2Q9>(*p, Window{nullptr, nullptr, {0, 0, 0}});
     return step_any<D2Q9>(cs, cur, false, f_out, true, VSX_PHASE_STREAM, s);

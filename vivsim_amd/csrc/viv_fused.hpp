@@ -264,9 +264,9 @@ k_viv_fused(StepCtx c_arg, VsxIB2D ib, const typename SRC::T *__restrict__ src, 
 // IB kernel of the pipelined form: step gstep's IB, pulling the window from the
 // hand-off buffer (written by the previous step's bulk pass, WSRC = kWinHalo;
 // it waits for that pass's flag) or, for the first step of a run, reading the
-// caller's post-streaming FP32 field in place (kWinLocal).  One block of 512
-// (M <= 512) or 1024 threads; its LDS and registers do not constrain the bulk
-// kernel's occupancy.
+// caller's post-streaming FP32 field in place (kWinLocal).  One block of 1024
+// threads; it runs on a compute unit of its own (the IB stream's CU mask), so
+// its LDS and registers do not constrain the bulk kernel's occupancy.
 template <int WSRC>
 __global__ __launch_bounds__(kIbThreads) void k_ib_pipe(StepCtx c_arg, VsxIB2D ib, const float *__restrict__ src,
                                                        int step) {
@@ -413,7 +413,7 @@ int fused_mode_any(const StepCtx &c, const VsxIB2D &ib, const void *src, void *d
 inline int launch_ib_pipe(const StepCtx &c, const VsxIB2D &ib, const float *src, bool first, int step,
                           hipStream_t s) {
   const size_t lds = 4 * (size_t)ib_atomic_lds_words(ib.win);
-  const int nt = ib.n_markers <= kIbThreads / 2 ? kIbThreads / 2 : kIbThreads;
+  const int nt = kIbThreads;
   if (first) hipLaunchKernelGGL((k_ib_pipe<kWinLocal>), dim3(1), dim3(nt), lds, s, c, ib, src, step);
   else hipLaunchKernelGGL((k_ib_pipe<kWinHalo>), dim3(1), dim3(nt), lds, s, c, ib, src, step);
   VSX_CHECK_LAUNCH();
