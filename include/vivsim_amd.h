@@ -364,6 +364,17 @@ int vsx_ib_multi_direct_forcing(const float *grid_u, int32_t c, int64_t n_grid, 
                                 const int32_t *I, int32_t m, int32_t s, const float *u_target,
                                 const float *ds_arr, float ds, int32_t n_iter, float *grid_force,
                                 float *marker_force, float *work, void *stream);
+/* No counterpart in the reference: vsx_ib_multi_direct_forcing without the work
+ * argument and without any device allocation, for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops).  Its scratch (ub, F, dF)
+ * is a fixed per-device array of the library sized for m * c <= 65536 marker
+ * components (768 KiB, held until the process ends); larger inputs return
+ * VSX_ERR_VALUE.  Every call on one device shares that array, so calls that
+ * overlap on one device (concurrent streams or threads) race. */
+int vsx_ib_multi_direct_forcing_noalloc(const float *grid_u, int32_t c, int64_t n_grid, const float *W,
+                                        const int32_t *I, int32_t m, int32_t s, const float *u_target,
+                                        const float *ds_arr, float ds, int32_t n_iter, float *grid_force,
+                                        float *marker_force, void *stream);
 /* ib/geometry.py:6-43 : coords [m][2] */
 int vsx_ib_get_area(const float *coords, int32_t m, float *out, void *stream);
 int vsx_ib_get_ds(const float *coords, int32_t m, int32_t closed, float *out, void *stream);

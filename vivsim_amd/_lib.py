@@ -95,6 +95,7 @@ _SIGS = {
     "vsx_ib_interpolate": "pilppiip p",
     "vsx_ib_spread": "pppilppii p",
     "vsx_ib_multi_direct_forcing": "pilppiipp fippp p",
+    "vsx_ib_multi_direct_forcing_noalloc": "pilppiipp fipp p",
     "vsx_ib_get_area": "pip p",
     "vsx_ib_get_ds": "piip p",
     "vsx_ib3d_get_triangle_areas": "pipip p",

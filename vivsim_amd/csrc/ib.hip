@@ -1,6 +1,8 @@
 // Standalone immersed-boundary, geometry and structural kernels behind the
 // pure-function API of vivsim.ib, vivsim.ib3d and vivsim.dyn.
 #include <cmath>
+#include <mutex>
+#include <vector>
 
 #include "common.hpp"
 #include "ib_common.hpp"
@@ -365,12 +367,12 @@ extern "C" int vsx_ib_spread(const float *mv, const float *grid_in, float *out, 
   return VSX_OK;
 }
 
-extern "C" int vsx_ib_multi_direct_forcing(const float *grid_u, int32_t c, int64_t ng, const float *W, const int32_t *I,
-                                           int32_t m, int32_t s, const float *U, const float *ds_arr, float ds,
-                                           int32_t n_iter, float *grid_force, float *marker_force, float *work,
-                                           void *stream) {
-  VSX_REQUIRE(c > 0 && ng > 0 && m > 0 && s > 0 && n_iter >= 0, "bad multi_direct_forcing sizes");
-  VSX_REQUIRE(work && grid_force && marker_force, "null buffer");
+namespace vsx {
+
+// ib/mdf.py:35-64 on caller buffers: work holds ub, F, dF ([m][c] each).
+static int mdf_run(const float *grid_u, int32_t c, int64_t ng, const float *W, const int32_t *I, int32_t m, int32_t s,
+                   const float *U, const float *ds_arr, float ds, int32_t n_iter, float *grid_force,
+                   float *marker_force, float *work, void *stream) {
   const int64_t mc = (int64_t)m * c, ms = (int64_t)m * s;
   float *ub = work, *F = work + mc, *dF = work + 2 * mc;
   L1(k_fill, (int64_t)c * ng, grid_force, 0.f, (int64_t)c * ng);
@@ -386,6 +388,56 @@ extern "C" int vsx_ib_multi_direct_forcing(const float *grid_u, int32_t c, int64
   L1(k_spread, ms, F, grid_force, c, ng, W, I, m, s);
   L1(k_neg, mc, F, marker_force, mc);
   return VSX_OK;
+}
+
+// Scratch of vsx_ib_multi_direct_forcing_noalloc: ub, F and dF for up to
+// kMdfScratchMC marker components (markers x components) each, 768 KiB per
+// device, held until the process ends.
+constexpr int64_t kMdfScratchMC = 65536;
+static __device__ __align__(256) float scratch[3 * kMdfScratchMC];
+
+// Device address of `scratch` on the current device (each device holds its own
+// copy), looked up once per device with that device current.  In host code the
+// name `scratch` is only a handle for this lookup, never a device address.
+static int scratch_ptr(float **out) {
+  static std::mutex mu;
+  static std::vector<float *> by_dev;
+  int dev = 0;
+  VSX_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  if ((int)by_dev.size() <= dev) by_dev.resize(dev + 1, nullptr);
+  if (!by_dev[dev]) {
+    void *p = nullptr;
+    VSX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    by_dev[dev] = static_cast<float *>(p);
+  }
+  *out = by_dev[dev];
+  return VSX_OK;
+}
+
+}  // namespace vsx
+
+extern "C" int vsx_ib_multi_direct_forcing(const float *grid_u, int32_t c, int64_t ng, const float *W, const int32_t *I,
+                                           int32_t m, int32_t s, const float *U, const float *ds_arr, float ds,
+                                           int32_t n_iter, float *grid_force, float *marker_force, float *work,
+                                           void *stream) {
+  VSX_REQUIRE(c > 0 && ng > 0 && m > 0 && s > 0 && n_iter >= 0, "bad multi_direct_forcing sizes");
+  VSX_REQUIRE(work && grid_force && marker_force, "null buffer");
+  return mdf_run(grid_u, c, ng, W, I, m, s, U, ds_arr, ds, n_iter, grid_force, marker_force, work, stream);
+}
+
+extern "C" int vsx_ib_multi_direct_forcing_noalloc(const float *grid_u, int32_t c, int64_t ng, const float *W,
+                                                   const int32_t *I, int32_t m, int32_t s, const float *U,
+                                                   const float *ds_arr, float ds, int32_t n_iter, float *grid_force,
+                                                   float *marker_force, void *stream) {
+  VSX_REQUIRE(c > 0 && ng > 0 && m > 0 && s > 0 && n_iter >= 0, "bad multi_direct_forcing sizes");
+  VSX_REQUIRE((int64_t)m * c <= kMdfScratchMC, "multi_direct_forcing_noalloc: %lld marker components exceed the "
+              "scratch limit of %lld", (long long)m * c, (long long)kMdfScratchMC);
+  VSX_REQUIRE(grid_force && marker_force, "null buffer");
+  float *work = nullptr;
+  const int rc = scratch_ptr(&work);
+  if (rc) return rc;
+  return mdf_run(grid_u, c, ng, W, I, m, s, U, ds_arr, ds, n_iter, grid_force, marker_force, work, stream);
 }
 
 extern "C" int vsx_ib_get_area(const float *p, int32_t m, float *out, void *stream) {
