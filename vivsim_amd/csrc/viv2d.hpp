@@ -19,6 +19,18 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Lane exchange of a 32-bit value from lane `src` (ds_bpermute).  The source
+// lane index is recomputed at each use: shared between distant exchanges it
+// would hold a register across the whole IB workgroup (or a spill slot).
+__device__ __forceinline__ float lane_get(float v, int src) {
+  asm volatile("" : "+v"(src));
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(v)));
+}
+__device__ __forceinline__ int lane_get(int v, int src) {
+  asm volatile("" : "+v"(src));
+  return __builtin_amdgcn_ds_bpermute(src << 2, v);
+}
+
 // Window origin of step t from the body state before the step:
 // int32(IB_X0 + d) truncated (examples/2d/vortex_induced_vibration.py:104-105),
 // and that origin clamped to the grid as lax.dynamic_slice clamps it.
@@ -436,14 +448,17 @@ __device__ __forceinline__ void seg_spread(float2 *grid, const float *rec, int M
   }
 }
 
-// Bounded wait of thread 0 until *flag >= target (relaxed agent-scope loads:
-// an acquire per poll would invalidate the L2 every time; callers that read
-// data published with the flag follow the wait with an agent-scope acquire
-// fence).  On time-out the error flag ib.win_origin[3] is set and the wait ends.
+// Bounded wait of thread 0 until *flag >= target, polling with relaxed
+// agent-scope loads every SLEEP x 64 cycles (an acquire per poll would
+// invalidate the L2 every time; data published with the flag is read with
+// ld_agent, see the hand-off rules of common.hpp).  Waits that are long anyway
+// poll less often, which keeps hundreds of pollers off the flag's memory
+// channel.  On time-out the error flag ib.win_origin[3] is set and the wait ends.
+template <int SLEEP = 2>
 __device__ __forceinline__ void spin_until(const VsxIB2D &ib, int32_t *flag, int target) {
   int spins = 0;
   while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-    __builtin_amdgcn_s_sleep(2);
+    __builtin_amdgcn_s_sleep(SLEEP);
     if (++spins > kSpinLimit) {
       __hip_atomic_store(&ib.win_origin[3], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       break;
@@ -882,17 +897,22 @@ __device__ __forceinline__ unsigned long long to_fix(float v) {
 }
 __device__ __forceinline__ float from_fix(unsigned long long q) { return (float)((double)(long long)q * (1.0 / kFixScale)); }
 
-// accumulation grid: cell k holds (x, y) as int64 words [2k, 2k + 1]
-__device__ __forceinline__ void acc_add(float *lds, int k, float vx, float vy) {
+// accumulation grid: two planes of SS int64 words, x components then y
+// components (cell k at words k and SS + k).  Eight-byte elements per plane
+// spread the lanes of one atomic over twice as many LDS banks as interleaved
+// (x, y) pairs would.
+__device__ __forceinline__ void acc_add(float *lds, int SS, int k, float vx, float vy) {
   unsigned long long *g = reinterpret_cast<unsigned long long *>(lds);
-  atomicAdd(&g[2 * k], to_fix(vx));
-  atomicAdd(&g[2 * k + 1], to_fix(vy));
+  atomicAdd(&g[k], to_fix(vx));
+  atomicAdd(&g[SS + k], to_fix(vy));
 }
-__device__ __forceinline__ float2 acc_get(const float *lds, int k) {
-  const ulonglong2 q = reinterpret_cast<const ulonglong2 *>(lds)[k];
-  return make_float2(from_fix(q.x), from_fix(q.y));
+__device__ __forceinline__ float2 acc_get(const float *lds, int SS, int k) {
+  const unsigned long long *g = reinterpret_cast<const unsigned long long *>(lds);
+  return make_float2(from_fix(g[k]), from_fix(g[SS + k]));
 }
 __host__ __device__ inline int ib_atomic_lds_words(int S) { return 4 * S * S; }
+// SPLIT layout (pipelined IB kernel): velocity, grid, mark array, cell list
+__host__ __device__ inline int ib_split_lds_words(int S) { return 6 * S * S + (S * S + 3) / 4 + (S * S + 1) / 2; }
 
 // Thread t of the IB workgroup works on marker mk and stencil rows
 // [r0, r0 + rows): with M <= nt / 2 two threads share a marker (rows 0-1 and
@@ -901,11 +921,18 @@ __host__ __device__ inline int ib_atomic_lds_words(int S) { return 4 * S * S; }
 // atomic mostly address different cells.
 struct IbLane {
   int mk, r0, rows, tpm;
+  // r0 recomputed from the thread index where it is used: a value kept live
+  // across the IB workgroup's phases would take a register (or a spill slot)
+  __device__ __forceinline__ int first_row() const {
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return tpm == 2 ? 2 * (t & 1) : 0;
+  }
 };
 __device__ __forceinline__ IbLane ib_lane(int M) {
   const int tid = threadIdx.x, nt = blockDim.x, nwave = nt >> 6, lane = tid & 63, wave = tid >> 6;
   IbLane l;
-  l.tpm = 2 * M <= nt ? 2 : 1;
+  l.tpm = __builtin_amdgcn_readfirstlane(2 * M <= nt ? 2 : 1);   // uniform: kept in a scalar register
   if (l.tpm == 2) {
     l.mk = (lane >> 1) * nwave + wave;
     l.r0 = 2 * (lane & 1);
@@ -920,18 +947,22 @@ __device__ __forceinline__ IbLane ib_lane(int M) {
 
 // Partial interpolation over this thread's rows (gather rule of ib/stencil.py:54-78
 // for flat indices outside the window), summed over the threads of the marker.
+// INSIDE (uniform over the wave): every stencil index of every lane lies in the
+// window, so a row is a base address plus constant offsets.
+template <bool ACC, bool INSIDE>
 __device__ __forceinline__ float2 interp_rows(const float *grid, int base, int S, const IbLane &l, const float (&wr)[4],
-                                              const float (&wy)[4], bool acc_grid) {
+                                              const float (&wy)[4]) {
   const int SS = S * S;
   asm volatile("" : "+v"(base));   // indices recomputed at every call (register budget)
   float sx = 0.f, sy = 0.f;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     if (i < l.rows) {
+      const int rb = base + i * S;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int k = gather_index32(base + i * S + j, SS);
-        const float2 v = acc_grid ? acc_get(grid, k) : reinterpret_cast<const float2 *>(grid)[k];
+        const int k = INSIDE ? rb + j : gather_index32(rb + j, SS);
+        const float2 v = ACC ? acc_get(grid, SS, k) : reinterpret_cast<const float2 *>(grid)[k];
         const float w = wr[i] * wy[j];
         sx += w * v.x;
         sy += w * v.y;
@@ -940,14 +971,21 @@ __device__ __forceinline__ float2 interp_rows(const float *grid, int base, int S
     }
   }
   if (l.tpm == 2) {
-    sx += __shfl_xor(sx, 1, 64);
-    sy += __shfl_xor(sy, 1, 64);
+    const int p = (int)(threadIdx.x & 63) ^ 1;
+    sx += lane_get(sx, p);
+    sy += lane_get(sy, p);
   }
   return make_float2(sx, sy);
+}
+template <bool ACC>
+__device__ __forceinline__ float2 interp_rows(const float *grid, int base, int S, const IbLane &l, const float (&wr)[4],
+                                              const float (&wy)[4], bool inside) {
+  return inside ? interp_rows<ACC, true>(grid, base, S, l, wr, wy) : interp_rows<ACC, false>(grid, base, S, l, wr, wy);
 }
 
 // Spread of (fx, fy) over this thread's rows (scatter rule of ib/stencil.py:106-110:
 // flat indices outside the window are dropped).
+template <bool INSIDE>
 __device__ __forceinline__ void spread_rows(float *grid, int base, int S, const IbLane &l, const float (&wr)[4],
                                             const float (&wy)[4], float fx, float fy) {
   const int SS = S * S;
@@ -955,16 +993,92 @@ __device__ __forceinline__ void spread_rows(float *grid, int base, int S, const 
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     if (i < l.rows) {
+      const int rb = base + i * S;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int k = scatter_index32(base + i * S + j, SS);
+        const int k = INSIDE ? rb + j : scatter_index32(rb + j, SS);
         if (k >= 0) {
           const float w = wr[i] * wy[j];
-          acc_add(grid, k, fx * w, fy * w);
+          acc_add(grid, SS, k, fx * w, fy * w);
         }
       }
     }
   }
+}
+__device__ __forceinline__ void spread_rows(float *grid, int base, int S, const IbLane &l, const float (&wr)[4],
+                                            const float (&wy)[4], float fx, float fy, bool inside) {
+  if (inside) spread_rows<true>(grid, base, S, l, wr, wy, fx, fy);
+  else spread_rows<false>(grid, base, S, l, wr, wy, fx, fy);
+}
+
+// Peskin 4-point kernel (ib/kernels.py) without branches: the same float32
+// expressions as delta_kernel, both sides evaluated and one selected.
+__device__ __forceinline__ float peskin4(float r) {
+  const float a = fabsf(r);
+  const bool in = a < 1.f;
+  const float q = sqrtf(in ? 1.f + 4.f * a - 4.f * (a * a) : -7.f + 12.f * a - 4.f * (a * a));
+  const float w = (in ? 3.f - 2.f * a + q : 5.f - 2.f * a - q) * 0.125f;
+  return a > 2.f ? 0.f : w;
+}
+
+// Stencil weights of this thread's rows (wr: x nodes r0 .. r0 + rows - 1,
+// zero beyond) and of the marker's four y nodes; with two threads per marker
+// each computes two of the y weights and takes the other two from its
+// partner lane.  Nodes ref - 1 .. ref + 2, ref = floor(x), weights phi(node - x)
+// in float32 (ib/stencil.py:7-51).
+template <class PHI>
+__device__ __forceinline__ void ib_weights(PHI phi, const IbLane &l, bool live, float rx, float ry, float (&wr)[4],
+                                           float (&wy)[4]) {
+  const int r0 = l.first_row();
+  const int xref = (int)floorf(rx), yref = (int)floorf(ry);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) wr[i] = (live && i < l.rows) ? phi((float)(xref - 1 + r0 + i) - rx) : 0.f;
+  if (l.tpm == 2) {
+    const float a = phi((float)(yref - 1 + r0) - ry), b = phi((float)(yref + r0) - ry);
+    const int p = (int)(threadIdx.x & 63) ^ 1;
+    const float pa = lane_get(a, p), pb = lane_get(b, p);
+    wy[0] = r0 == 0 ? a : pa;
+    wy[1] = r0 == 0 ? b : pb;
+    wy[2] = r0 == 0 ? pa : a;
+    wy[3] = r0 == 0 ? pb : b;
+  } else {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) wy[j] = phi((float)(yref - 1 + j) - ry);
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) wy[j] = live ? wy[j] : 0.f;
+}
+
+// Compaction of the marked window cells: thread t counts the marks of cells
+// [t C, t C + C), C = ceil(n / blockDim.x); a block-wide exclusive scan of the
+// counts gives where its cells go in `list` (ascending cell order).  Returns
+// the number of marked cells (every thread); ends with a barrier.
+__device__ __forceinline__ int mark_scan(const uint8_t *mark, uint16_t *list, int n, int *s_scan) {
+  const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6, nwave = (nt + 63) >> 6;
+  const int C = (n + nt - 1) / nt, k0 = tid * C, k1 = min(k0 + C, n);
+  int cnt = 0;
+  for (int k = k0; k < k1; ++k) cnt += mark[k];
+  int inc = cnt;   // inclusive scan over the wave
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int v = lane_get(inc, lane >= o ? lane - o : lane);
+    if (lane >= o) inc += v;
+  }
+  if (lane == 63) s_scan[wave] = inc;
+  __syncthreads();
+  // wave totals: lane w holds wave w's; sums over the waves below this one and over all
+  const int wt = lane < nwave ? s_scan[lane] : 0;
+  int below = lane < wave ? wt : 0, total = wt;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    below += lane_get(below, lane ^ o);
+    total += lane_get(total, lane ^ o);
+  }
+  int off = inc - cnt + below;
+  for (int k = k0; k < k1; ++k)
+    if (mark[k]) list[off++] = (uint16_t)k;
+  __syncthreads();
+  return total;
 }
 
 // One workgroup per body (examples/2d/vortex_induced_vibration.py:99-137),
@@ -985,19 +1099,29 @@ constexpr int kWinLocal = 1;   // read src in place (a post-streaming state)
 constexpr int kWinHalo = 2;    // pull from the FP32 hand-off buffer `halo` [9][S + 4][S + 4] whose origin
                                // is the previous step's window origin - 2 (ib.win_origin[0..1])
 
-template <class SRC, int WSRC, int UW, int TAG>
+template <class SRC, int WSRC, int UW, int TAG, bool SPLIT = false>
 __device__ __forceinline__ void ib_atomic(const StepCtx &c, const VsxIB2D &ib, const typename SRC::T *__restrict__ src,
                                           const float *halo, int hidx, int gstep, int32_t *go_flag,
-                                          const double *body_in, double *body_out, float *lds) {
+                                          const double *body_in, double *body_out, float *lds,
+                                          int32_t *wait_flag = nullptr, int wait_target = 0) {
   const int S = ib.win, SS = S * S, M = ib.n_markers;
   __shared__ double s_body[6];
   __shared__ int s_org[6];
   __shared__ double s_red[2][kIbThreads / 64];
+  __shared__ int s_scan[kIbThreads / 64 + 1];
   const int tid = threadIdx.x, nt = blockDim.x;
   const IbLane l = ib_lane(M);
   const bool live = l.mk < M;
 
+  // round trip 1: body state and marker inputs
   if (tid < 6) s_body[tid] = body_in[tid];
+  double mx = 0.0, my = 0.0;
+  float ds2 = 0.f;
+  if (live) {
+    mx = ib.marker_x[l.mk];
+    my = ib.marker_y[l.mk];
+    ds2 = (ib.ds_arr ? ib.ds_arr[l.mk] : ib.ds) * 2.0f;
+  }
   __syncthreads();
   if (tid == 0) {
     int o[4];
@@ -1017,99 +1141,133 @@ __device__ __forceinline__ void ib_atomic(const StepCtx &c, const VsxIB2D &ib, c
     ib.win_origin[0] = o[2];
     ib.win_origin[1] = o[3];
   }
+  // LDS: window velocity (2 SS words) and accumulation grid (4 SS words), the
+  // grid over the velocity (zeroed once the velocity is dead) or, SPLIT,
+  // behind it (zeroed before the wait); the mark array (one byte per window
+  // cell) and the cell list behind the velocity (over the grid's start) or,
+  // SPLIT, behind the grid
+  float *acc = SPLIT ? lds + 2 * SS : lds;
+  uint8_t *mark = reinterpret_cast<uint8_t *>(lds + (SPLIT ? 6 : 2) * SS);
+  uint16_t *list = reinterpret_cast<uint16_t *>(lds + (SPLIT ? 6 : 2) * SS + (SS + 3) / 4);
+  for (int k = tid; k < (SS + 3) / 4; k += nt) reinterpret_cast<uint32_t *>(mark)[k] = 0u;
   __syncthreads();
   const int x0 = s_org[0], y0 = s_org[1], xc = s_org[2], yc = s_org[3];
 
-  // window velocity (float2 per cell over the start of the accumulation grid)
-  float2 *ugrid = reinterpret_cast<float2 *>(lds);
-  if constexpr (WSRC == kWinHalo) {
-    const int E = S + 4, EE = E * E, dx = xc - s_org[4], dy = yc - s_org[5];
-    const bool ok = dx >= 1 && dx <= 3 && dy >= 1 && dy <= 3;
-    for (int k = tid; k < SS; k += nt) {
-      float f[9], rho, u[3];
-      const int i = k / S + dx, j = k % S + dy;
+  // marker position relative to the window origin; mark the window cells the
+  // interpolation of this thread's rows reads (zero weights included)
+  float rx = 0.f, ry = 0.f;
+  if (live) {
+    rx = (float)(mx + s_body[4] - (double)x0);
+    ry = (float)(my + s_body[5] - (double)y0);
+    const int b0 = ((int)floorf(rx) - 1 + l.first_row()) * S + ((int)floorf(ry) - 1);
 #pragma unroll
-      for (int q = 0; q < 9; ++q)
-        f[q] = ok ? ld_agent(&halo[q * EE + (i - D2Q9::cx[q]) * E + (j - D2Q9::cy[q])]) : D2Q9::w[q];
-      macroscopic<D2Q9>(f, rho, u);
-      ugrid[k] = make_float2(u[0], u[1]);
-    }
-  } else {
-    constexpr bool PULL = WSRC == kWinPull;
-    for (int base = 0; base < SS; base += UW * nt) {
-      float fu[UW][9];
-      bool fu_on[UW];
-      window_pull<SRC, PULL, UW>(c, src, base, S, xc, yc, fu, fu_on);
-      window_store<UW>(ugrid, base, fu, fu_on);
-    }
-    if (PULL && c.p.n_walls > 0) {
-      // window cells on a domain wall (rare: only when the window touches a
-      // wall), inline: an out-of-line call would make the caller save live
-      // registers.  Fused steps have no wall corners (every wall cell is of a
-      // single-wall class).
-      for (int k = tid; k < SS; k += nt) {
-        const int co[3] = {xc + k / S, yc + k % S, 0};
-        const int cls = wall_class<D2Q9>(c, co);
-        if (cls >= 0) {
-          float f[9], rho, u[3];
-          single_wall_state<D2Q9, SRC>(c, src, co, cls, f);
-          macroscopic<D2Q9>(f, rho, u);
-          ugrid[k] = make_float2(u[0], u[1]);
-        }
+    for (int i = 0; i < 4; ++i) {
+      if (i < l.rows) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) mark[gather_index32(b0 + i * S + j, SS)] = 1;
       }
     }
   }
-  // marker inputs (after the window pull: held across it they would take registers there)
-  double mx = 0.0, my = 0.0;
-  float ds2 = 0.f;
-  if (live) {
-    mx = ib.marker_x[l.mk];
-    my = ib.marker_y[l.mk];
-    ds2 = (ib.ds_arr ? ib.ds_arr[l.mk] : ib.ds) * 2.0f;
+  __syncthreads();
+  // compaction: thread t owns cells [t C, t C + C); block-wide exclusive scan of the counts
+  const int n_list = mark_scan(mark, list, SS, s_scan);
+
+  // stencil of this thread's marker and rows: SPLIT before the wait, else after
+  // the pull (whose registers it would take)
+  float wr[4], wy[4];
+  int base = 0;
+  bool inside = true;
+  auto stencil = [&]() {
+    if (ib.kernel == VSX_KERNEL_PESKIN_4PT) ib_weights(peskin4, l, live, rx, ry, wr, wy);
+    else ib_weights([&](float r) { return delta_kernel(ib.kernel, r); }, l, live, rx, ry, wr, wy);
+    // flat window index of this thread's first stencil point, and whether every
+    // stencil index of the wave lies inside the window
+    base = live ? ((int)floorf(rx) - 1 + l.first_row()) * S + ((int)floorf(ry) - 1) : 0;
+    inside = __all(base >= 0 && base + (l.rows - 1) * S + 3 < SS);
+  };
+  if constexpr (SPLIT) {
+    stencil();
+    for (int k = tid; k < SS; k += nt) reinterpret_cast<float4 *>(acc)[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  // the hand-off of the window populations (SPLIT: the zeroing above is
+  // ordered before the spread by the barriers below)
+  if (wait_flag) {
+    if (tid == 0) spin_until(ib, wait_flag, wait_target);
+    __syncthreads();
+  }
+
+  // round trip 2: window velocity of the marked cells (float2 per cell over the
+  // start of the accumulation grid; the other cells are never read)
+  float2 *ugrid = reinterpret_cast<float2 *>(lds);
+  for (int idx = tid; idx < n_list; idx += nt) {
+    const int k = list[idx];
+    float f[9], rho, u[3];
+    if constexpr (WSRC == kWinHalo) {
+      // a window outside the hand-off buffer (an error, flagged above) reads
+      // the buffer's corner instead: every load stays in bounds and all nine
+      // are issued back to back
+      const int E = S + 4, EE = E * E;
+      const int dx = min(max(xc - s_org[4], 1), 3), dy = min(max(yc - s_org[5], 1), 3);
+      const int i = k / S + dx, j = k % S + dy;
+#pragma unroll
+      for (int q = 0; q < 9; ++q) f[q] = ld_agent(&halo[q * EE + (i - D2Q9::cx[q]) * E + (j - D2Q9::cy[q])]);
+    } else {
+      const int co[3] = {xc + k / S, yc + k % S, 0};
+      // cells on a domain wall are left to the loop below
+      if (WSRC == kWinPull && c.p.n_walls > 0 && wall_class<D2Q9>(c, co) >= 0) continue;
+      if constexpr (WSRC == kWinPull) pull<D2Q9, SRC>(c, src, co[0], co[1], 0, f);
+      else local<D2Q9, SRC>(c, src, cell_of<D2Q9>(c, co[0], co[1], 0), f);
+    }
+    macroscopic<D2Q9>(f, rho, u);
+    ugrid[k] = make_float2(u[0], u[1]);
+  }
+  if (WSRC == kWinPull && c.p.n_walls > 0) {
+    // marked cells on a domain wall (only when the window touches a wall;
+    // fused steps have no wall corners, so every wall cell is of a
+    // single-wall class), in a loop of their own: inline in the loop above
+    // the wall code's registers would add to the pull's
+    for (int idx = tid; idx < n_list; idx += nt) {
+      const int k = list[idx];
+      const int co[3] = {xc + k / S, yc + k % S, 0};
+      const int cls = wall_class<D2Q9>(c, co);
+      if (cls >= 0) {
+        float f[9], rho, u[3];
+        single_wall_state<D2Q9, SRC>(c, src, co, cls, f);
+        macroscopic<D2Q9>(f, rho, u);
+        ugrid[k] = make_float2(u[0], u[1]);
+      }
+    }
   }
   __syncthreads();
   if (tid == 0 && go_flag) __hip_atomic_store(go_flag, gstep + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 
-  // stencil of this thread's marker and rows
-  float wr[4], wy[4];
-  int base = 0;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) wr[i] = wy[i] = 0.f;
-  if (live) {
-    const float rx = (float)(mx + s_body[4] - (double)x0);
-    const float ry = (float)(my + s_body[5] - (double)y0);
-    const Axis<4> ax = axis_stencil<4>(ib.kernel, rx), ay = axis_stencil<4>(ib.kernel, ry);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      wr[i] = l.r0 == 0 ? ax.w[i] : (i < 2 ? ax.w[i + 2] : 0.f);
-      wy[i] = ay.w[i];
-    }
-    base = (ax.ref - 1 + l.r0) * S + (ay.ref - 1);   // flat window index of this thread's first stencil point
-  }
+  if constexpr (!SPLIT) stencil();
 
   // initial interpolation of the window velocity
   float ub0x = 0.f, ub0y = 0.f;
   if (live) {
-    const float2 v = interp_rows(lds, base, S, l, wr, wy, false);
+    const float2 v = interp_rows<false>(lds, base, S, l, wr, wy, inside);
     ub0x = v.x;
     ub0y = v.y;
   }
   float ubx = ub0x, uby = ub0y, Fx = 0.f, Fy = 0.f;
   const float Ux = (float)s_body[2], Uy = (float)s_body[3];
-  __syncthreads();   // the window velocity is dead from here; the grid accumulates the spread force
-  for (int k = tid; k < ib_atomic_lds_words(S); k += nt) lds[k] = 0.f;
-  __syncthreads();
+  if constexpr (!SPLIT) {
+    __syncthreads();   // the window velocity is dead from here; the grid accumulates the spread force
+    for (int k = tid; k < SS; k += nt) reinterpret_cast<float4 *>(acc)[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    __syncthreads();
+  }
 
   // multi-direct forcing (ib/mdf.py:45-64)
   for (int it = 0; it < ib.n_iter; ++it) {
     const float sfx = (Ux - ubx) * ds2, sfy = (Uy - uby) * ds2;
     Fx += sfx;
     Fy += sfy;
-    if (live) spread_rows(lds, base, S, l, wr, wy, sfx, sfy);
+    if (live) spread_rows(acc, base, S, l, wr, wy, sfx, sfy, inside);
     __syncthreads();
     if (it + 1 == ib.n_iter) break;
     if (live) {
-      const float2 v = interp_rows(lds, base, S, l, wr, wy, true);
+      const float2 v = interp_rows<true>(acc, base, S, l, wr, wy, inside);
       ubx = ub0x + 0.5f * v.x;
       uby = ub0y + 0.5f * v.y;
     }
@@ -1118,14 +1276,14 @@ __device__ __forceinline__ void ib_atomic(const StepCtx &c, const VsxIB2D &ib, c
 
   // force window -> global
   for (int k = tid; k < SS; k += nt) {
-    const float2 g = acc_get(lds, k);
+    const float2 g = acc_get(acc, SS, k);
     st_agent(&ib.g_win[k], g.x);
     st_agent(&ib.g_win[SS + k], g.y);
   }
 
   // body force h = sum(-F) (FP64), one thread per marker
   double hx = 0.0, hy = 0.0;
-  if (live && l.r0 == 0) {
+  if (live && l.first_row() == 0) {
     hx = -(double)Fx;
     hy = -(double)Fy;
   }
@@ -1295,6 +1453,11 @@ inline int validate_ib2d(const VsxLbm *p, const VsxIB2D *ib) {
   VSX_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
   const size_t need = 4 * (size_t)ib_lds_words(ib->win, ib->n_markers, ib->n_markers <= kIbThreads) + 2048;
   VSX_REQUIRE(need <= (size_t)max_lds, "IB window %d needs %zu B of LDS, device allows %d", ib->win, need, max_lds);
+  if (ib->fused == 2) {
+    const size_t need_pipe = 4 * (size_t)ib_split_lds_words(ib->win) + 2048;
+    VSX_REQUIRE(need_pipe <= (size_t)max_lds, "pipelined IB kernel: window %d needs %zu B of LDS, device allows %d",
+                ib->win, need_pipe, max_lds);
+  }
   return VSX_OK;
 }
 

@@ -36,6 +36,8 @@
 namespace vsx {
 
 constexpr int kModeFull = 0, kModeFirst = 1, kModeLast = 2;
+// poll interval (x 64 cycles) of the bulk tiles waiting for the IB workgroup's window pull
+constexpr int kBulkSleep = 16;
 
 // Push-order store of a collided cell: f_i goes to x + c_i (periodic), which is
 // the reference's streamed field before the walls.  Destination cells are
@@ -136,11 +138,26 @@ k_viv_fused(StepCtx c_arg, VsxIB2D ib, const typename SRC::T *__restrict__ src, 
   const int gstep = ib.step0 + step, slot = gstep & 1;
   const double *body_in = ib.body + 6 * slot;
   const bool pipe = ib.fused == 2;
+  if (!pipe && blockIdx.x == 0) {
+    ib_atomic<SRC, MODE == kModeFirst ? kWinLocal : kWinPull, 2, 1>(c, ib, src, nullptr, step, gstep,
+                                                                     &ib.win_origin[6], body_in,
+                                                                     ib.body + 6 * (slot ^ 1), lds);
+    publish_ready(ib, gstep + 1);
+    return;
+  }
   __shared__ int s_org[2];
   if (threadIdx.x == 0) {
+    // one-launch form: the body state before this step was written by the
+    // previous launch (plain loads); pipelined form: by the IB kernel of the
+    // previous step on the other stream (agent-scope loads)
     double b[6];
-    b[4] = __hip_atomic_load(const_cast<double *>(body_in + 4), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    b[5] = __hip_atomic_load(const_cast<double *>(body_in + 5), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (pipe) {
+      b[4] = __hip_atomic_load(const_cast<double *>(body_in + 4), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      b[5] = __hip_atomic_load(const_cast<double *>(body_in + 5), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      b[4] = body_in[4];
+      b[5] = body_in[5];
+    }
     int o[4];
     window_origin(c, ib, b, o);
     s_org[0] = o[2];
@@ -159,13 +176,6 @@ k_viv_fused(StepCtx c_arg, VsxIB2D ib, const typename SRC::T *__restrict__ src, 
   } else {
     window_tiles(T, org[0], org[1], ib.win, wx0, wx1, wy0, wy1);
   }
-  if (!pipe && blockIdx.x == 0) {
-    ib_atomic<SRC, MODE == kModeFirst ? kWinLocal : kWinPull, 2, 1>(c, ib, src, nullptr, step, gstep,
-                                                                     &ib.win_origin[6], body_in,
-                                                                     ib.body + 6 * (slot ^ 1), lds);
-    publish_ready(ib, gstep + 1);
-    return;
-  }
   const int bid = pipe ? (int)blockIdx.x : (int)blockIdx.x - 1;   // tile-block index
   const int n_win = (wx1 - wx0 + 1) * (wy1 - wy0 + 1);
   if constexpr (PAIR) {
@@ -181,7 +191,7 @@ k_viv_fused(StepCtx c_arg, VsxIB2D ib, const typename SRC::T *__restrict__ src, 
       const int b[3] = {has_b ? tb[0] * T.TX + (int)threadIdx.x / T.TY : c.dims[0],
                         has_b ? tb[1] * T.TY + (int)threadIdx.x % T.TY : c.dims[1], 0};
       if (!pipe) {
-        if (threadIdx.x == 0) spin_until(ib, &ib.win_origin[6], gstep + 1);
+        if (threadIdx.x == 0) spin_until<kBulkSleep>(ib, &ib.win_origin[6], gstep + 1);
         __syncthreads();
       }
       const bool va = a[0] < c.dims[0] && a[1] < c.dims[1], vb = b[0] < c.dims[0] && b[1] < c.dims[1];
@@ -240,7 +250,7 @@ k_viv_fused(StepCtx c_arg, VsxIB2D ib, const typename SRC::T *__restrict__ src, 
   }
   // bulk tile (one-launch form): wait (bounded) until the IB workgroup has pulled the window
   if (!pipe) {
-    if (threadIdx.x == 0) spin_until(ib, &ib.win_origin[6], gstep + 1);
+    if (threadIdx.x == 0) spin_until<kBulkSleep>(ib, &ib.win_origin[6], gstep + 1);
     __syncthreads();
   }
   if (!valid) return;
@@ -273,13 +283,13 @@ __global__ __launch_bounds__(kIbThreads) void k_ib_pipe(StepCtx c_arg, VsxIB2D i
   const StepCtx &c = kernel_ctx();
   extern __shared__ float lds[];
   const int gstep = ib.step0 + step, slot = gstep & 1;
-  if constexpr (WSRC == kWinHalo) {
-    if (threadIdx.x == 0) spin_until(ib, &ib.win_origin[6], gstep);
-    __syncthreads();
-  }
   const float *halo = ib.halo + (size_t)((gstep - 1) & 1) * 9 * (ib.win + 4) * (ib.win + 4);
-  ib_atomic<Store<VSX_STORE_F32>, WSRC, 2, 2>(c, ib, src, halo, step, gstep, nullptr, ib.body + 6 * slot,
-                                               ib.body + 6 * (slot ^ 1), lds);
+  // everything up to the window pull (body state, marks, cell list, stencil,
+  // zeroed grid) is done while the previous step's bulk pass still streams;
+  // the pull waits for that pass's hand-off flag
+  ib_atomic<Store<VSX_STORE_F32>, WSRC, 2, 2, true>(c, ib, src, halo, step, gstep, nullptr, ib.body + 6 * slot,
+                                                     ib.body + 6 * (slot ^ 1), lds,
+                                                     WSRC == kWinHalo ? &ib.win_origin[6] : nullptr, gstep);
   publish_ready(ib, gstep + 1);
 }
 
@@ -412,7 +422,7 @@ int fused_mode_any(const StepCtx &c, const VsxIB2D &ib, const void *src, void *d
 // IB kernel of step `step` of the pipelined form (first: read the caller's field src in place)
 inline int launch_ib_pipe(const StepCtx &c, const VsxIB2D &ib, const float *src, bool first, int step,
                           hipStream_t s) {
-  const size_t lds = 4 * (size_t)ib_atomic_lds_words(ib.win);
+  const size_t lds = 4 * (size_t)ib_split_lds_words(ib.win);
   const int nt = kIbThreads;
   if (first) hipLaunchKernelGGL((k_ib_pipe<kWinLocal>), dim3(1), dim3(nt), lds, s, c, ib, src, step);
   else hipLaunchKernelGGL((k_ib_pipe<kWinHalo>), dim3(1), dim3(nt), lds, s, c, ib, src, step);

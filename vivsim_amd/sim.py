@@ -285,7 +285,8 @@ class VIV2D:
     least 2 cells inside the grid; a run that breaks this raises on the next
     call); pipelined=False: one launch per step, the IB workgroup beside the
     bulk tiles (any window position).  pipelined=None picks the pipelined form
-    when the initial window is at least 3 cells inside the grid."""
+    when the initial window is at least 3 cells inside the grid and the IB
+    kernel's LDS (27 S^2 bytes for window side S) fits the device."""
 
     def __init__(self, case: VIVCase, storage="f32", fused=True, pipelined=None):
         self.case = case
@@ -298,8 +299,9 @@ class VIV2D:
         self.ib = self._b.ib
         if pipelined is None:
             S = case.ib_size
+            lds = 4 * (6 * S * S + (S * S + 3) // 4 + (S * S + 1) // 2) + 2048
             pipelined = min(case.ib_x0, case.ib_y0) >= 3 and case.ib_x0 + S + 3 <= case.nx and \
-                case.ib_y0 + S + 3 <= case.ny
+                case.ib_y0 + S + 3 <= case.ny and lds <= torch.cuda.get_device_properties(dev).shared_memory_per_block
         self.ib.fused = (2 if pipelined else 1) if fused else 0
         self._watch = _TimeoutWatch([self.win_origin[3:4]])
 
