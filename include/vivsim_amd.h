@@ -127,6 +127,10 @@ typedef struct VsxIB2D {
   float *halo;         /* device [2][9][win + 4][win + 4] FP32 (fused == 2): post-collision
                           populations around the window, handed from the bulk pass of a step
                           to the IB kernel of the next (half (step0 + t) & 1 by step t)     */
+  int32_t *status;     /* optional pinned host int32 (hipHostMalloc) or NULL: the kernels also
+                          store their error codes there (1 = a bounded wait ran out, 2 = the
+                          pipelined window left the hand-off buffer), so a host can check a
+                          run without copying win_origin[3] back                            */
 } VsxIB2D;
 
 /* 3D immersed body held fixed in a D3Q19 VsxLbm step (flow past a sphere).

@@ -288,3 +288,23 @@ def test_viv_zero_iterations_is_plain_step(sim):
     e = assert_close(a, to_np(b), 1e-6, "viv n_iter=0 vs plain")
     say(f"max rel diff {e:.2e}")
     assert torch.all(hh == 0)
+
+
+def test_pipelined_window_jump_is_reported(sim):
+    """A body moving two or more cells in a step moves the IB window out of the
+    hand-off buffer of the pipelined step: the kernels report it (every access
+    stays in bounds) through the pinned status word, ``check`` raises, the next
+    run raises without a sync, and ``set_body`` clears the report."""
+    pc = small_case(sim.VIVCase)
+    f0 = dev(small_case(S.VIV2D).initial_f(np.float32))
+    solver = sim.VIV2D(pc, pipelined=True)
+    solver.set_body(v=(0.0, 2.5))
+    say("pipelined window jump")
+    f, _, _ = solver.run(f0, 4, history=False)
+    with pytest.raises(RuntimeError, match="pipelined VIV step"):
+        solver.check()
+    with pytest.raises(RuntimeError, match="pipelined VIV step"):
+        solver.run(f0, 1)
+    solver.set_body(v=(0.0, 1e-3))
+    solver.run(f0, 4)
+    solver.check()

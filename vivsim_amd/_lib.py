@@ -54,7 +54,7 @@ class VsxIB2D(ctypes.Structure):
                 ("dt", c_double), ("gamma", c_double), ("beta", c_double),
                 ("body", c_void_p), ("g_win", c_void_p), ("hist_d", c_void_p), ("hist_h", c_void_p),
                 ("win_origin", c_void_p), ("step0", c_int32), ("fused", c_int32),
-                ("reserved", c_int32), ("halo", c_void_p)]
+                ("reserved", c_int32), ("halo", c_void_p), ("status", c_void_p)]
 
 
 class VsxVivCase(ctypes.Structure):
@@ -210,7 +210,8 @@ def device():
 
 
 def stream():
-    return c_void_p(torch.cuda.current_stream().cuda_stream)
+    """The current HIP stream of the current device (raw handle, no Stream object)."""
+    return c_void_p(torch._C._cuda_getCurrentRawStream(torch.cuda.current_device()))
 
 
 def ptr(t):

@@ -578,9 +578,8 @@ int viv_adjoint(const VsxLbm *p, const VsxIB2D *ib, const VsxAdj2D *a, const flo
   rc = validate_adj(p, ib, a);
   if (rc) return rc;
   VSX_REQUIRE(f0 && wa && wb && f_tmp, "null buffer");
-  int dev = 0, max_lds = 0;
-  VSX_HIP(hipGetDevice(&dev));
-  VSX_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  int max_lds = 0;
+  if (int rc_ = max_lds_per_block(&max_lds)) return rc_;
   VSX_REQUIRE(sizeof(float) * (size_t)adj_lds_layout(ib->win, ib->n_markers, 1, false).total + 2048 <= (size_t)max_lds,
               "IB window %d too large for the adjoint workgroup", ib->win);
   hipStream_t s = as_stream(stream);

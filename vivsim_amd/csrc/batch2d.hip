@@ -196,9 +196,8 @@ int batch_prepare(const VsxVivCase *cases, int n, void *table, int32_t *clock, V
   VSX_REQUIRE(cases && table && clock && out, "null argument");
   VSX_REQUIRE(n > 0 && n < (1 << 11), "case count %d out of range (1..2047)", n);
   const VsxLbm &p0 = cases[0].lbm;
-  int dev = 0, max_lds = 0;
-  VSX_HIP(hipGetDevice(&dev));
-  VSX_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  int max_lds = 0;
+  if (int rc_ = max_lds_per_block(&max_lds)) return rc_;
   std::vector<BatchCase> tab(n);
   std::vector<uint32_t> outer, band;
   int64_t pops = 0;

@@ -61,6 +61,32 @@ constexpr int kMaxDevices = 64;
 PipeRes g_pipe[kMaxDevices];
 std::mutex g_pipe_mu;
 
+int max_lds_per_block(int *out) {
+  static int cache[64] = {0};
+  int dev = 0;
+  VSX_HIP(hipGetDevice(&dev));
+  VSX_REQUIRE(dev >= 0 && dev < 64, "device index %d out of range", dev);
+  if (cache[dev] == 0) {
+    int v = 0;
+    VSX_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    cache[dev] = v;
+  }
+  *out = cache[dev];
+  return VSX_OK;
+}
+
+int host_device_pointer(void *host, void **dev_ptr) {
+  static thread_local void *last_host = nullptr, *last_dev = nullptr;
+  if (host != last_host || !last_dev) {
+    void *d = nullptr;
+    VSX_HIP(hipHostGetDevicePointer(&d, host, 0));
+    last_host = host;
+    last_dev = d;
+  }
+  *dev_ptr = last_dev;
+  return VSX_OK;
+}
+
 int pipe_res(PipeRes **out) {
   int dev = 0;
   VSX_HIP(hipGetDevice(&dev));
@@ -68,26 +94,14 @@ int pipe_res(PipeRes **out) {
   std::lock_guard<std::mutex> lock(g_pipe_mu);
   PipeRes &r = g_pipe[dev];
   if (!r.s) {
-    int n_cu = 0;
-    VSX_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
-    std::vector<uint32_t> ib_mask((n_cu + 31) / 32, 0u), bulk_mask((n_cu + 31) / 32, 0u);
-    for (int k = 0; k < n_cu; ++k) {
-      if (k == n_cu - 1) ib_mask[k / 32] |= 1u << (k % 32);
-      else bulk_mask[k / 32] |= 1u << (k % 32);
-    }
-    r.masked = n_cu > 1 &&
-               hipExtStreamCreateWithCUMask(&r.s, (uint32_t)ib_mask.size(), ib_mask.data()) == hipSuccess &&
-               hipExtStreamCreateWithCUMask(&r.b, (uint32_t)bulk_mask.size(), bulk_mask.data()) == hipSuccess;
-    if (!r.masked) {
-      (void)hipGetLastError();
-      if (r.s) (void)hipStreamDestroy(r.s);
-      if (r.b) (void)hipStreamDestroy(r.b);
-      VSX_HIP(hipStreamCreateWithFlags(&r.s, hipStreamNonBlocking));
-      VSX_HIP(hipStreamCreateWithFlags(&r.b, hipStreamNonBlocking));
-    }
-    VSX_HIP(hipEventCreateWithFlags(&r.fork, hipEventDisableTiming));
-    VSX_HIP(hipEventCreateWithFlags(&r.join_s, hipEventDisableTiming));
-    VSX_HIP(hipEventCreateWithFlags(&r.join_b, hipEventDisableTiming));
+    VSX_HIP(hipStreamCreateWithFlags(&r.s, hipStreamNonBlocking));
+    VSX_HIP(hipStreamCreateWithFlags(&r.b, hipStreamNonBlocking));
+    // ordering only: every kernel launch already acquires at its start and
+    // releases at its end, so the events need no cache fence of their own
+    const unsigned ev = hipEventDisableTiming | hipEventDisableSystemFence;
+    VSX_HIP(hipEventCreateWithFlags(&r.fork, ev));
+    VSX_HIP(hipEventCreateWithFlags(&r.join_s, ev));
+    VSX_HIP(hipEventCreateWithFlags(&r.join_b, ev));
   }
   *out = &r;
   return VSX_OK;

@@ -20,19 +20,22 @@ int fail(int code, const char *fmt, ...);
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
 // Streams of the pipelined steps (default priority, non-blocking): `s` runs the
-// IB kernels on a compute-unit mask of one CU, `b` the bulk passes on all the
-// other CUs, so the latency-bound IB never shares its CU with streaming
-// blocks and never waits for one to drain; events fork both from the caller's
-// stream and join them back.  One set per device, created on first use and
-// kept for the process (if the runtime refuses CU masks, both are plain
-// streams).  Calls that use them from several host threads at once on one
-// device are not supported.
+// IB kernels; the 3D step also runs its bulk passes on `b`, forked from the
+// caller's stream and joined back with the events (the 2D step keeps its bulk
+// passes on the caller's stream and needs no events).  One set per device,
+// created on first use and kept for the process.  Calls that use them from
+// several host threads at once on one device are not supported.
 struct PipeRes {
   hipStream_t s = nullptr, b = nullptr;
   hipEvent_t fork = nullptr, join_s = nullptr, join_b = nullptr;
-  bool masked = false;
 };
 int pipe_res(PipeRes **out);
+
+// Largest LDS allocation of one workgroup on the current device (queried once
+// per device).
+int max_lds_per_block(int *out);
+// Device address of pinned host memory (the last translation is kept).
+int host_device_pointer(void *host, void **dev);
 
 // Checks the launch (and any earlier asynchronous error) of the kernels just queued.
 #define VSX_CHECK_LAUNCH()                                                              \

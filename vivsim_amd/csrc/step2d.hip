@@ -55,12 +55,20 @@ int fused_full(const StepCtx &c, const VsxIB2D &ib, const void *src, void *dst, 
 }
 
 
-int viv_run(const VsxLbm *p, const VsxIB2D *ib, const float *f_in, float *f_out, void *wa, void *wb, int n,
+int viv_run(const VsxLbm *p, const VsxIB2D *ib_in, const float *f_in, float *f_out, void *wa, void *wb, int n,
             void *stream) {
   int rc = validate_lbm<D2Q9>(p);
   if (rc) return rc;
-  rc = validate_ib2d(p, ib);
+  rc = validate_ib2d(p, ib_in);
   if (rc) return rc;
+  // the kernels' view of the IB parameters: the status word as a device address
+  VsxIB2D ib_dev = *ib_in;
+  if (ib_dev.status) {
+    void *d = nullptr;
+    if ((rc = host_device_pointer(ib_in->status, &d))) return rc;
+    ib_dev.status = static_cast<int32_t *>(d);
+  }
+  const VsxIB2D *ib = &ib_dev;
   VSX_REQUIRE(f_in && f_out && wa, "null buffer");
   VSX_REQUIRE(n >= 0, "negative step count");
   VSX_REQUIRE(n <= 1 || wb, "need two work buffers for more than one step");
@@ -79,26 +87,25 @@ int viv_run(const VsxLbm *p, const VsxIB2D *ib, const float *f_in, float *f_out,
   const void *cur = f_in;
   void *bufs[2] = {wa, wb};
   if (fused && ib->fused == 2) {
-    // pipelined: IB kernels on the second stream, bulk passes on the caller's
+    // pipelined: bulk passes on the caller's stream, IB kernels on a second
+    // stream.  No stream events: the first IB kernel waits (in the kernel) for
+    // the first bulk pass to start, each later one for the previous pass's
+    // hand-off, and every bulk pass for its IB kernel's force window, so the
+    // caller's stream is complete only after every IB kernel of the run has
+    // written its last result (IB kernels of consecutive runs keep their order
+    // on the second stream).
     VSX_REQUIRE(ib->halo, "the pipelined VIV step needs the hand-off buffer ib->halo");
     PipeRes *r = nullptr;
     if ((rc = pipe_res(&r))) return rc;
-    VSX_HIP(hipEventRecord(r->fork, s));
-    VSX_HIP(hipStreamWaitEvent(r->s, r->fork, 0));
-    VSX_HIP(hipStreamWaitEvent(r->b, r->fork, 0));
     for (int t = 0; t < n; ++t) {
       if ((rc = launch_ib_pipe(c, *ib, f_in, t == 0, t, r->s))) return rc;
       void *dst = (push && t == n - 1) ? f_out : bufs[t & 1];
-      if (t == 0) rc = fused_first(c, *ib, cur, dst, t, r->b);
-      else if (push && t == n - 1) rc = fused_last(c, *ib, cur, dst, t, r->b);
-      else rc = fused_full(c, *ib, cur, dst, t, r->b);
+      if (t == 0) rc = fused_first(c, *ib, cur, dst, t, s);
+      else if (push && t == n - 1) rc = fused_last(c, *ib, cur, dst, t, s);
+      else rc = fused_full(c, *ib, cur, dst, t, s);
       if (rc) return rc;
       cur = dst;
     }
-    VSX_HIP(hipEventRecord(r->join_s, r->s));
-    VSX_HIP(hipEventRecord(r->join_b, r->b));
-    VSX_HIP(hipStreamWaitEvent(s, r->join_s, 0));
-    VSX_HIP(hipStreamWaitEvent(s, r->join_b, 0));
     if (push) return walls_in_place<D2Q9>(c, f_out, s);
     const StepCtx cs = make_ctx<D2Q9>(*p, Window{nullptr, nullptr, {0, 0, 0}});
     return step_any<D2Q9>(cs, cur, false, f_out, true, VSX_PHASE_STREAM, s);

@@ -448,6 +448,13 @@ __device__ __forceinline__ void seg_spread(float2 *grid, const float *rec, int M
   }
 }
 
+// Error code of a fused step: into win_origin[3] and, when the caller gave
+// one, the pinned host word ib.status.
+__device__ __forceinline__ void set_ib_error(const VsxIB2D &ib, int code) {
+  __hip_atomic_store(&ib.win_origin[3], code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (ib.status) __hip_atomic_store(ib.status, code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // Bounded wait of thread 0 until *flag >= target, polling with relaxed
 // agent-scope loads every SLEEP x 64 cycles (an acquire per poll would
 // invalidate the L2 every time; data published with the flag is read with
@@ -460,7 +467,7 @@ __device__ __forceinline__ void spin_until(const VsxIB2D &ib, int32_t *flag, int
   while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
     __builtin_amdgcn_s_sleep(SLEEP);
     if (++spins > kSpinLimit) {
-      __hip_atomic_store(&ib.win_origin[3], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      set_ib_error(ib, 1);
       break;
     }
   }
@@ -1113,8 +1120,22 @@ __device__ __forceinline__ void ib_atomic(const StepCtx &c, const VsxIB2D &ib, c
   const IbLane l = ib_lane(M);
   const bool live = l.mk < M;
 
+  // the first IB kernel of a pipelined run (START) waits for the run's first
+  // bulk pass to begin, i.e. for the caller's earlier work on its stream, and
+  // then reads what that work may have written (body state, the field) with
+  // agent-scope loads
+  constexpr bool START = SPLIT && WSRC == kWinLocal;
+  if constexpr (START) {
+    if (wait_flag) {
+      if (tid == 0) spin_until(ib, wait_flag, wait_target);
+      __syncthreads();
+    }
+  }
   // round trip 1: body state and marker inputs
-  if (tid < 6) s_body[tid] = body_in[tid];
+  if (tid < 6)
+    s_body[tid] = START ? __hip_atomic_load(const_cast<double *>(body_in + tid), __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT)
+                        : body_in[tid];
   double mx = 0.0, my = 0.0;
   float ds2 = 0.f;
   if (live) {
@@ -1134,12 +1155,14 @@ __device__ __forceinline__ void ib_atomic(const StepCtx &c, const VsxIB2D &ib, c
       const int ex = ib.win_origin[0] - 2, ey = ib.win_origin[1] - 2;
       const bool ok = o[2] - ex >= 1 && o[2] - ex <= 3 && o[3] - ey >= 1 && o[3] - ey <= 3 && o[2] >= 2 &&
                       o[3] >= 2 && o[2] + S + 2 <= c.dims[0] && o[3] + S + 2 <= c.dims[1];
-      if (!ok) __hip_atomic_store(&ib.win_origin[3], 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (!ok) set_ib_error(ib, 2);
       s_org[4] = ex;
       s_org[5] = ey;
     }
-    ib.win_origin[0] = o[2];
-    ib.win_origin[1] = o[3];
+    // agent-scope stores: read by the next IB kernel and by the caller's stream,
+    // which in the pipelined form is not ordered after this kernel's end
+    __hip_atomic_store(&ib.win_origin[0], o[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&ib.win_origin[1], o[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   // LDS: window velocity (2 SS words) and accumulation grid (4 SS words), the
   // grid over the velocity (zeroed once the velocity is dead) or, SPLIT,
@@ -1191,7 +1214,7 @@ __device__ __forceinline__ void ib_atomic(const StepCtx &c, const VsxIB2D &ib, c
   }
   // the hand-off of the window populations (SPLIT: the zeroing above is
   // ordered before the spread by the barriers below)
-  if (wait_flag) {
+  if (!START && wait_flag) {
     if (tid == 0) spin_until(ib, wait_flag, wait_target);
     __syncthreads();
   }
@@ -1215,8 +1238,15 @@ __device__ __forceinline__ void ib_atomic(const StepCtx &c, const VsxIB2D &ib, c
       const int co[3] = {xc + k / S, yc + k % S, 0};
       // cells on a domain wall are left to the loop below
       if (WSRC == kWinPull && c.p.n_walls > 0 && wall_class<D2Q9>(c, co) >= 0) continue;
-      if constexpr (WSRC == kWinPull) pull<D2Q9, SRC>(c, src, co[0], co[1], 0, f);
-      else local<D2Q9, SRC>(c, src, cell_of<D2Q9>(c, co[0], co[1], 0), f);
+      if constexpr (WSRC == kWinPull) {
+        pull<D2Q9, SRC>(c, src, co[0], co[1], 0, f);
+      } else if constexpr (START) {
+        const float *p = reinterpret_cast<const float *>(src) + cell_of<D2Q9>(c, co[0], co[1], 0);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) f[q] = ld_agent(p + (size_t)q * c.n);
+      } else {
+        local<D2Q9, SRC>(c, src, cell_of<D2Q9>(c, co[0], co[1], 0), f);
+      }
     }
     macroscopic<D2Q9>(f, rho, u);
     ugrid[k] = make_float2(u[0], u[1]);
@@ -1315,12 +1345,12 @@ __device__ __forceinline__ void ib_atomic(const StepCtx &c, const VsxIB2D &ib, c
     }
     for (int k = 0; k < 6; ++k) __hip_atomic_store(&body_out[k], out[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (ib.hist_d && hidx >= 0) {
-      ib.hist_d[2 * hidx] = out[4];
-      ib.hist_d[2 * hidx + 1] = out[5];
+      __hip_atomic_store(&ib.hist_d[2 * hidx], out[4], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&ib.hist_d[2 * hidx + 1], out[5], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     if (ib.hist_h && hidx >= 0) {
-      ib.hist_h[2 * hidx] = h[0];
-      ib.hist_h[2 * hidx + 1] = h[1];
+      __hip_atomic_store(&ib.hist_h[2 * hidx], h[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&ib.hist_h[2 * hidx + 1], h[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
 }
@@ -1448,9 +1478,8 @@ inline int validate_ib2d(const VsxLbm *p, const VsxIB2D *ib) {
   VSX_REQUIRE(ib->marker_x && ib->marker_y && ib->body && ib->g_win && ib->win_origin, "null IB buffer");
   VSX_REQUIRE(ib->mass + ib->gamma * ib->dt * ib->damping + ib->beta * ib->dt * ib->dt * ib->stiffness != 0.0,
               "singular Newmark system");
-  int dev = 0, max_lds = 0;
-  VSX_HIP(hipGetDevice(&dev));
-  VSX_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  int max_lds = 0;
+  if (int rc_ = max_lds_per_block(&max_lds)) return rc_;
   const size_t need = 4 * (size_t)ib_lds_words(ib->win, ib->n_markers, ib->n_markers <= kIbThreads) + 2048;
   VSX_REQUIRE(need <= (size_t)max_lds, "IB window %d needs %zu B of LDS, device allows %d", ib->win, need, max_lds);
   if (ib->fused == 2) {

@@ -138,6 +138,12 @@ k_viv_fused(StepCtx c_arg, VsxIB2D ib, const typename SRC::T *__restrict__ src, 
   const int gstep = ib.step0 + step, slot = gstep & 1;
   const double *body_in = ib.body + 6 * slot;
   const bool pipe = ib.fused == 2;
+  if constexpr (MODE == kModeFirst) {
+    // pipelined run: the caller's earlier work on this stream is complete; let
+    // the run's first IB kernel start
+    if (pipe && blockIdx.x == 0 && threadIdx.x == 0)
+      __hip_atomic_store(&ib.win_origin[7], gstep + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
   if (!pipe && blockIdx.x == 0) {
     ib_atomic<SRC, MODE == kModeFirst ? kWinLocal : kWinPull, 2, 1>(c, ib, src, nullptr, step, gstep,
                                                                      &ib.win_origin[6], body_in,
@@ -287,9 +293,12 @@ __global__ __launch_bounds__(kIbThreads) void k_ib_pipe(StepCtx c_arg, VsxIB2D i
   // everything up to the window pull (body state, marks, cell list, stencil,
   // zeroed grid) is done while the previous step's bulk pass still streams;
   // the pull waits for that pass's hand-off flag
+  // (the first IB kernel of a run waits for the run's first bulk pass to start:
+  // ib.win_origin[7] = step0 + 1)
   ib_atomic<Store<VSX_STORE_F32>, WSRC, 2, 2, true>(c, ib, src, halo, step, gstep, nullptr, ib.body + 6 * slot,
                                                      ib.body + 6 * (slot ^ 1), lds,
-                                                     WSRC == kWinHalo ? &ib.win_origin[6] : nullptr, gstep);
+                                                     WSRC == kWinHalo ? &ib.win_origin[6] : &ib.win_origin[7],
+                                                     WSRC == kWinHalo ? gstep : gstep + 1);
   publish_ready(ib, gstep + 1);
 }
 
@@ -299,8 +308,9 @@ __global__ __launch_bounds__(kIbThreads) void k_ib_pipe(StepCtx c_arg, VsxIB2D i
 // carry the walls applied before this one), and rewrites the wall cell.  The
 // wall must not need the pre-streaming populations.
 template <class L>
-__global__ __launch_bounds__(256) void k_wall_line(StepCtx c_arg, float *__restrict__ f, int k) {
+__global__ __launch_bounds__(256) void k_wall_line(StepCtx c_arg, float *__restrict__ f, int k0) {
   const StepCtx &c = kernel_ctx();
+  const int k = k0 + (int)blockIdx.y;   // one wall per grid row when the walls are independent
   const VsxWall &w = c.p.walls[k];
   const int a = L::w_axis[w.loc];
   const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -334,15 +344,32 @@ __global__ __launch_bounds__(256) void k_wall_line(StepCtx c_arg, float *__restr
   for (int i = 0; i < L::Q; ++i) f[(size_t)i * c.n + cell] = fw[i];
 }
 
-// Walls of a pushed field, in list order (one launch per wall).
+// Walls of a pushed field, in list order (one launch per wall).  Walls on
+// distinct faces of one axis, with at least 4 cells between them, neither
+// share a cell nor read one another's cells: then all go in one launch.
 template <class L>
 int walls_in_place(const StepCtx &c, float *f, hipStream_t s) {
-  for (int k = 0; k < c.p.n_walls; ++k) {
-    const int a = L::w_axis[c.p.walls[k].loc];
-    int64_t nodes = 1;
-    for (int b = 0; b < L::D; ++b)
-      if (b != a) nodes *= c.dims[b];
-    hipLaunchKernelGGL((k_wall_line<L>), dim3((unsigned)((nodes + 255) / 256)), dim3(256), 0, s, c, f, k);
+  const int n = c.p.n_walls;
+  if (n == 0) return VSX_OK;
+  const int a0 = L::w_axis[c.p.walls[0].loc];
+  bool together = c.dims[a0] >= 4;
+  for (int k = 0; k < n; ++k)
+    for (int j = 0; j < k; ++j)
+      together = together && L::w_axis[c.p.walls[k].loc] == a0 && c.p.walls[k].loc != c.p.walls[j].loc;
+  int64_t nodes = 1;
+  for (int b = 0; b < L::D; ++b)
+    if (b != a0) nodes *= c.dims[b];
+  const unsigned gx = (unsigned)((nodes + 255) / 256);
+  if (together) {
+    hipLaunchKernelGGL((k_wall_line<L>), dim3(gx, (unsigned)n), dim3(256), 0, s, c, f, 0);
+  } else {
+    for (int k = 0; k < n; ++k) {
+      const int a = L::w_axis[c.p.walls[k].loc];
+      int64_t nk = 1;
+      for (int b = 0; b < L::D; ++b)
+        if (b != a) nk *= c.dims[b];
+      hipLaunchKernelGGL((k_wall_line<L>), dim3((unsigned)((nk + 255) / 256)), dim3(256), 0, s, c, f, k);
+    }
   }
   VSX_CHECK_LAUNCH();
   return VSX_OK;

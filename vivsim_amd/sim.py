@@ -18,6 +18,7 @@ exactly like consecutive ``boundary_*`` calls.
 """
 
 import ctypes
+from ctypes import c_void_p
 import math
 from dataclasses import dataclass
 
@@ -225,15 +226,20 @@ class _TimeoutWatch:
             self.event.synchronize()
         elif not self.event.query():
             return
-        codes = [int(self.host[k]) for k in range(len(self.flags))]
-        if any(c == 2 for c in codes):
-            bad = [k for k, c in enumerate(codes) if c == 2]
-            raise RuntimeError(f"pipelined VIV step, case(s) {bad}: the IB window came closer than 2 cells to the "
-                               "grid edge or moved by more than a cell in a step; run with pipelined=False")
-        bad = [k for k, c in enumerate(codes) if c != 0]
-        if bad:
-            raise RuntimeError(f"fused VIV step: a tile of case(s) {bad} timed out waiting for the IB workgroup "
-                               "(the force window of that step was not applied)")
+        _raise_ib_codes([int(self.host[k]) for k in range(len(self.flags))])
+
+
+def _raise_ib_codes(codes):
+    """The error codes of fused steps (1: a bounded wait ran out, 2: the
+    pipelined window left the hand-off buffer), one per case."""
+    if any(c == 2 for c in codes):
+        bad = [k for k, c in enumerate(codes) if c == 2]
+        raise RuntimeError(f"pipelined VIV step, case(s) {bad}: the IB window came closer than 2 cells to the "
+                           "grid edge or moved by more than a cell in a step; run with pipelined=False")
+    bad = [k for k, c in enumerate(codes) if c != 0]
+    if bad:
+        raise RuntimeError(f"fused VIV step: a tile of case(s) {bad} timed out waiting for the IB workgroup "
+                           "(the force window of that step was not applied)")
 
 
 class _Body2D:
@@ -267,12 +273,19 @@ class _Body2D:
         ib.g_win = self.g_win.data_ptr()
         ib.win_origin = self.win_origin.data_ptr()
         ib.halo = self.halo.data_ptr()
+        # error word the kernels store into directly (pinned host memory): read
+        # without a copy or a sync
+        self.status = torch.zeros(1, dtype=torch.int32, pin_memory=True)
+        self.status_np = self.status.numpy()
+        ib.status = self.status.data_ptr()
         self.ib = ib
 
     def set(self, a, v, d, slot=0):
         self.body.zero_()
         self.body[slot].copy_(torch.tensor(list(a) + list(v) + list(d), dtype=torch.float64))
         self.win_origin.zero_()
+        torch.cuda.current_stream().synchronize()
+        self.status_np[0] = 0
 
 
 class VIV2D:
@@ -303,7 +316,7 @@ class VIV2D:
             pipelined = min(case.ib_x0, case.ib_y0) >= 3 and case.ib_x0 + S + 3 <= case.nx and \
                 case.ib_y0 + S + 3 <= case.ny and lds <= torch.cuda.get_device_properties(dev).shared_memory_per_block
         self.ib.fused = (2 if pipelined else 1) if fused else 0
-        self._watch = _TimeoutWatch([self.win_origin[3:4]])
+        self._run_args = None   # bound entry point and the pointers that stay fixed between runs
 
     def set_body(self, a=(0.0, 0.0), v=(0.0, 0.0), d=(0.0, 0.0)):
         """Reset the structure state (and the step counter of the body slots)."""
@@ -311,23 +324,29 @@ class VIV2D:
         self.t_total = 0
 
     def state(self):
-        self._watch.poll(wait=True)
         b = self.body[self.t_total & 1].cpu().numpy()
+        self._poll()
         return b[0:2].copy(), b[2:4].copy(), b[4:6].copy()   # a, v, d
+
+    def _poll(self):
+        code = int(self._b.status_np[0])
+        if code:
+            _raise_ib_codes([code])
 
     def check(self):
         """Raise if a tile of a fused step timed out waiting for the IB workgroup
         (waits for the runs queued so far).  ``run`` raises the same error, without
-        waiting, once a time-out of an earlier run is visible on the host."""
-        self._watch.after_run()
-        self._watch.poll(wait=True)
+        waiting, once the kernels' error word of an earlier run has reached the host."""
+        torch.cuda.current_stream().synchronize()
+        self._poll()
 
     def run(self, f, n_steps, out=None, history=True):
         """Advance n_steps; returns (f_out, d_hist [n,2], h_hist [n,2]) (histories FP64 on device).
         ``history`` may also be a pair of caller-owned FP64 [>= n_steps, 2] device
         tensors (d, h) to fill, which spares the allocations in a loop of runs."""
-        self._watch.poll()
-        f = f32(f)
+        self._poll()
+        if not (isinstance(f, torch.Tensor) and f.is_cuda and f.dtype == torch.float32 and f.is_contiguous()):
+            f = f32(f)
         out = torch.empty_like(f) if out is None else out
         hd = hh = None
         if isinstance(history, tuple):
@@ -339,12 +358,14 @@ class VIV2D:
             hh = torch.empty((n_steps, 2), dtype=torch.float64, device=f.device)
         self.ib.hist_d = None if hd is None else hd.data_ptr()
         self.ib.hist_h = None if hh is None else hh.data_ptr()
-        wa, wb = self.lbm.work()
+        if self._run_args is None:
+            wa, wb = self.lbm.work()
+            self._run_args = (L.lib().vsx_viv2d_run, ctypes.byref(self.lbm.p), ctypes.byref(self.ib),
+                              L.ptr(wa), L.ptr(wb))
+        fn, p, ib, wa, wb = self._run_args
         self.ib.step0 = self.t_total
-        L.call("vsx_viv2d_run", ctypes.byref(self.lbm.p), ctypes.byref(self.ib), L.ptr(f), L.ptr(out), L.ptr(wa),
-               L.ptr(wb), int(n_steps))
+        L.check(fn(p, ib, c_void_p(f.data_ptr()), c_void_p(out.data_ptr()), wa, wb, int(n_steps), L.stream()))
         self.t_total += int(n_steps)
-        self._watch.after_run()
         return out, hd, hh
 
 
@@ -563,6 +584,7 @@ class VIVBatch:
             ib = self.bodies[k].ib
             ib.hist_d, ib.hist_h = self.hist_d[k].data_ptr(), self.hist_h[k].data_ptr()
             vc[k].ib = ib
+            vc[k].ib.status = None   # the batch reads its error flags through its own watch
             vc[k].f_offset = self.offsets[k]
             vc[k].tile_y = self.tile_y[k]
             vc[k].hist_cap = self.hist_cap
