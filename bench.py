@@ -199,6 +199,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=2000)
+    ap.add_argument("--only", choices=["grid_4096", "grid_4096_f16s", "grid_4096_e3m12s", "sphere3d", "batch64",
+                                       "adjoint"],
+                    help="profiling aid: run only this variant (no headline line) and print its result")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run 5 device-to-device copies of 256 MiB (calibration of HBM byte counters)")
     args = ap.parse_args()
@@ -222,6 +225,14 @@ def main():
     import torch
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
+
+    if args.only:
+        big = argparse.Namespace(**{**vars(args), "nx": 4096, "ny": 4096})
+        fns = {"grid_4096": lambda: run_gpu(big, "f32", 200, 20), "grid_4096_f16s": lambda: run_gpu(big, "f16s", 200, 20),
+               "grid_4096_e3m12s": lambda: run_gpu(big, "e3m12s", 200, 20), "sphere3d": lambda: run_sphere(100),
+               "batch64": lambda: run_batch(400), "adjoint": lambda: run_adjoint(args.adjoint_steps)}
+        print(json.dumps({args.only: fns[args.only]()}), flush=True)
+        return 0
 
     main_run = run_gpu(args, args.storage, args.steps, args.warmup)
     print(f"headline: {main_run['glups']:.3f} GLUPS, {main_run['ms_per_step'] * 1e3:.2f} us / step, "
