@@ -191,11 +191,14 @@ def small_sphere(cls, **kw):
 @pytest.mark.parametrize("variant", [dict(), dict(n_iter=3, collision="mrt", forcing="guo"),
                                      dict(n_iter=1, inlet="nebb", outlet="eq"), dict(n_iter=0)],
                          ids=["bgk_edm_2it", "mrt_guo_3it", "nebb_1it", "no_ib"])
-def test_sphere_matches_oracle(sim, variant):
+@pytest.mark.parametrize("pipelined", [False, True], ids=["sequential", "pipelined"])
+def test_sphere_matches_oracle(sim, variant, pipelined):
+    """Both launch forms: IB kernels then the bulk pass on the caller's stream,
+    or the IB of step t + 1 on a second stream beside the bulk pass of step t."""
     oc, pc = small_sphere(S.Sphere3D, **variant), small_sphere(sim.SphereCase, **variant)
     steps = 12
-    say(f"sphere {variant}")
-    solver = sim.Sphere3D(pc)
+    say(f"sphere {variant} pipelined={pipelined}")
+    solver = sim.Sphere3D(pc, pipelined=pipelined)
     f0 = oc.initial_f(np.float64)
     got, hh = solver.run(dev(f0), steps)
     W, I = oc.stencil(np.float32)

@@ -216,31 +216,33 @@ struct AdjIB {
   double inv_T;
 };
 
-// LDS of k_adj_ib in 4-byte words (S = window side, SS = S * S, M = markers;
-// grids are float2 [SS], x and y of a cell side by side):
-//   uw [SS] window velocity of f_t, G [SS] spread grid, KEEP: A [NIT][SS] the
-//   forward replay's spreads; then the binned-marker tables of the spread
-//   (viv2d.hpp): rec [max(10 M, SS + 1)], bend (zero-padded bin ends), list.
+// LDS of k_adj_ib in 4-byte words (S = window side, SS = S * S):
+//   uw [2 SS] float2 window velocity of f_t; acc [4 SS] the fixed-point spread
+//   grid (two int64 planes, x then y); KEEP: A [NIT][2 SS] float2, the forward
+//   replay's spreads; mark [SS bytes] the window cells the stencils read; list
+//   [SS uint16] those cells in ascending order.
 struct AdjLds {
-  int G, A, rec, bend, list, total;
+  int acc, A, mark, list, total;
 };
 __host__ __device__ inline AdjLds adj_lds_layout(int S, int M, int nit, bool keep) {
+  (void)M;
   const int SS = S * S;
   AdjLds L;
-  L.G = 2 * SS;
-  L.A = 4 * SS;
-  L.rec = ((keep ? 4 + 2 * nit : 4) * SS + 3) & ~3;
-  L.bend = (L.rec + imax(10 * M, SS + 1) + 3) & ~3;
-  L.list = L.bend + (bend_pad(S) + SS + 2) / 2;
-  L.total = L.list + imax(M, (SS + 1) / 2);
+  L.acc = 2 * SS;
+  L.A = 6 * SS;
+  L.mark = L.A + (keep ? 2 * nit * SS : 0);
+  L.list = L.mark + (SS + 3) / 4;
+  L.total = L.list + (SS + 1) / 2;
   return L;
 }
 
-// KEEP: the forward replay's spread grids A_k stay in LDS for the reverse loop,
-// else they are spread again.  Spreads: markers binned by stencil base cell
-// once per step, each spread a deterministic gather over the active window
-// cells (bin_spread); markers whose stencil leaves the window use LDS atomics
-// with the index rule of the operation (scatter or gather rule).
+// Reverse IB step.  Thread t takes marker mk and stencil rows [r0, r0 + rows)
+// as in the forward IB workgroup (two threads per marker when 2 M <= 1024);
+// only the window cells the stencils read are pulled, zeroed and converted;
+// every spread adds int64 fixed-point values in LDS (exact, independent of
+// the order of the additions), with the index rule of the operation for
+// stencils that leave the window.  KEEP: the forward replay's spread grids
+// A_k stay in LDS for the reverse loop, else they are spread again.
 template <int NIT, bool KEEP>
 __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const float *__restrict__ ft,
                                                  const float *__restrict__ gbar, float *__restrict__ fbar,
@@ -253,21 +255,31 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
   const size_t n = (size_t)P.nx * P.ny;
   const AdjLds lay = adj_lds_layout(S, M, NIT, KEEP);
   float2 *uw = reinterpret_cast<float2 *>(lds);            // window velocity of f_t
-  float2 *G = reinterpret_cast<float2 *>(lds + lay.G);     // spread grid
+  float *acc = lds + lay.acc;                              // spread grid (fixed point)
   float2 *Ak = reinterpret_cast<float2 *>(lds + lay.A);    // KEEP: [NIT] forward spreads
-  float *rec = lds + lay.rec;
-  int *cnt = reinterpret_cast<int *>(rec);
-  uint16_t *bend = reinterpret_cast<uint16_t *>(lds + lay.bend) + bend_pad(S);
-  int *blist = reinterpret_cast<int *>(lds + lay.list);
-  uint16_t *actl = reinterpret_cast<uint16_t *>(lds + lay.list);
+  uint8_t *mark = reinterpret_cast<uint8_t *>(lds + lay.mark);
+  uint16_t *list = reinterpret_cast<uint16_t *>(lds + lay.list);
   __shared__ double s_body[6], s_ab[6], s_misc[4];
   __shared__ double red[16];
-  __shared__ int s_org[4], s_wsum[kIbThreads / 64], s_irr, s_nirr, s_nact;
+  __shared__ int s_org[4], s_scan[kIbThreads / 64 + 1];
+  const IbLane l = ib_lane(M);
+  const bool act = l.mk < M;
+  const int m = l.mk;
+  const int partner = (int)(threadIdx.x & 63) ^ 1;
+
+  // round trip 1: body state, its adjoint, marker inputs
   if (tid < 6) {
     s_body[tid] = body[tid];
     s_ab[tid] = adj_body[tid];
   }
-  if (tid == 0) s_irr = s_nirr = 0;
+  double mxd = 0.0, myd = 0.0;
+  float ds2 = 0.f;
+  if (act) {
+    mxd = ib.marker_x[m];
+    myd = ib.marker_y[m];
+    ds2 = (ib.ds_arr ? ib.ds_arr[m] : ib.ds) * 2.0f;
+  }
+  for (int k = tid; k < (SS + 3) / 4; k += nt) reinterpret_cast<uint32_t *>(mark)[k] = 0u;
   __syncthreads();
   if (tid == 0) {
     // objective term and Newmark adjoint (dyn.py:5-51), per DOF
@@ -291,18 +303,51 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
       s_misc[k] = Hb;
     }
     s_misc[2] = kbar;
-    int o[4];
     const int x0 = (int)trunc(ib.win_x0 + s_body[4]), y0 = (int)trunc(ib.win_y0 + s_body[5]);
-    o[0] = x0;
-    o[1] = y0;
-    o[2] = min(max(x0, 0), P.nx - S);
-    o[3] = min(max(y0, 0), P.ny - S);
-    for (int k = 0; k < 4; ++k) s_org[k] = o[k];
+    s_org[0] = x0;
+    s_org[1] = y0;
+    s_org[2] = min(max(x0, 0), P.nx - S);
+    s_org[3] = min(max(y0, 0), P.ny - S);
   }
   __syncthreads();
   const int x0 = s_org[0], y0 = s_org[1], xc = s_org[2], yc = s_org[3];
-  // window velocity of f_t
-  for (int k = tid; k < SS; k += nt) {
+
+  // stencil of this thread's rows (weights and their derivatives); mark the cells it reads
+  float wr[4] = {0, 0, 0, 0}, dwr[4] = {0, 0, 0, 0}, wy[4] = {0, 0, 0, 0}, dwy[4] = {0, 0, 0, 0};
+  int base = 0;
+  if (act) {
+    const int r0 = l.first_row();
+    const float rx = (float)(mxd + s_body[4] - (double)x0);
+    const float ry = (float)(myd + s_body[5] - (double)y0);
+    const int refx = (int)floorf(rx), refy = (int)floorf(ry);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (i < l.rows) {
+        const float ax = (float)(refx - 1 + r0 + i) - rx;
+        wr[i] = delta_kernel(ib.kernel, ax);
+        dwr[i] = delta_kernel_deriv(ib.kernel, ax);
+      }
+      const float ay = (float)(refy - 1 + i) - ry;
+      wy[i] = delta_kernel(ib.kernel, ay);
+      dwy[i] = delta_kernel_deriv(ib.kernel, ay);
+    }
+    base = (refx - 1 + r0) * S + (refy - 1);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (i < l.rows) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) mark[gather_index32(base + i * S + j, SS)] = 1;
+      }
+    }
+  }
+  __syncthreads();
+  const int n_list = mark_scan(mark, list, SS, s_scan);   // ends with a barrier
+  const bool inside = __all(base >= 0 && base + (l.rows - 1) * S + 3 < SS);
+  const int base0 = base;
+
+  // round trip 2: window velocity of f_t at the marked cells
+  for (int idx = tid; idx < n_list; idx += nt) {
+    const int k = list[idx];
     const size_t cell = (size_t)(xc + k / S) * P.ny + (yc + k % S);
     float rho = 0.f, jx = 0.f, jy = 0.f;
 #pragma unroll
@@ -314,143 +359,114 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
     }
     uw[k] = make_float2(jx / rho, jy / rho);
   }
-  // marker of this thread (lane-major, as the forward) and its stencil + derivatives
-  const int m = (tid & 63) * (nt >> 6) + (tid >> 6);
-  const bool act = m < M;
-  float wx[4] = {0, 0, 0, 0}, wy[4] = {0, 0, 0, 0}, dwx[4] = {0, 0, 0, 0}, dwy[4] = {0, 0, 0, 0};
-  int base = 0;
-  bool regular = false;
-  float ds2 = 0.f;
-  if (act) {
-    const float rx = (float)(ib.marker_x[m] + s_body[4] - (double)x0);
-    const float ry = (float)(ib.marker_y[m] + s_body[5] - (double)y0);
-    const int refx = (int)floorf(rx), refy = (int)floorf(ry);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const float ax = (float)(refx - 1 + i) - rx, ay = (float)(refy - 1 + i) - ry;
-      wx[i] = delta_kernel(ib.kernel, ax);
-      wy[i] = delta_kernel(ib.kernel, ay);
-      dwx[i] = delta_kernel_deriv(ib.kernel, ax);
-      dwy[i] = delta_kernel_deriv(ib.kernel, ay);
-    }
-    base = (refx - 1) * S + (refy - 1);
-    ds2 = (ib.ds_arr ? ib.ds_arr[m] : ib.ds) * 2.0f;
-    regular = refx - 1 >= 0 && refx - 1 <= S - 4 && refy - 1 >= 0 && refy - 1 <= S - 4;
-    if (!regular) s_irr = 1;
-  }
-  // markers binned by base cell (count, scan, place, rank by marker index)
-  const bool binned = act && regular;
-  for (int k = tid; k <= SS; k += nt) cnt[k] = 0;
   __syncthreads();
-  if (binned) atomicAdd(&cnt[base], 1);
-  __syncthreads();
-  block_exclusive_scan(cnt, SS, s_wsum);
-  if (binned) blist[atomicAdd(&cnt[base], 1)] = m;
-  __syncthreads();
-  int my_pos = 0;
-  if (binned) {
-    const int s0 = base > 0 ? cnt[base - 1] : 0, e0 = cnt[base];
-    int rank = 0;
-    for (int i = s0; i < e0; ++i) rank += blist[i] < m;
-    my_pos = s0 + rank;
-  } else if (act) {
-    my_pos = M - 1 - atomicAdd(&s_nirr, 1);
-  }
-  for (int k = tid - bend_pad(S); k < SS; k += nt) bend[k] = k < 0 ? 0 : (uint16_t)cnt[k];
-  __syncthreads();   // counters and marker list are dead from here
-  if (binned) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      rec[i * M + my_pos] = wx[i];
-      rec[(4 + i) * M + my_pos] = wy[i];
-    }
-  }
-  const int n_act = active_cells(bend, actl, S, &s_nact);   // ends with a barrier
-  float2 *rforce = reinterpret_cast<float2 *>(rec + 8 * M);
 
   const float Ux = (float)s_body[2], Uy = (float)s_body[3];
-  float rxb = 0.f, ryb = 0.f;   // adjoint of the marker position relative to the window
-  // gather of a 2-component grid at the stencil: value sums and W-bar terms with
-  // coefficients (cx, cy); fetch(gi) returns the grid value of window cell gi
+  float rxb = 0.f, ryb = 0.f;   // this thread's part of the marker-position adjoint
+  auto pair_sum = [&](float &vx, float &vy) {
+    if (l.tpm == 2) {
+      vx += lane_get(vx, partner);
+      vy += lane_get(vy, partner);
+    }
+  };
+  // gather of a 2-component grid at this thread's rows: value sums (summed over
+  // the marker's threads) and the W-bar terms with coefficients (cx, cy)
+  // (the stencil indices are recomputed in every call: shared between calls
+  // they would stay live across the whole kernel)
   auto gather_with = [&](auto fetch, bool scatter_rule, float cx, float cy, float &vx, float &vy) {
     vx = vy = 0.f;
-    if (!act) return;
+    int base = base0;
+    asm volatile("" : "+v"(base));
+    if (act) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int p = base + i * S + j;
-        int gi;
-        if (scatter_rule) {
-          gi = scatter_index32(p, SS);
-          if (gi < 0) continue;
-        } else {
-          gi = gather_index32(p, SS);
-        }
-        const float w = wx[i] * wy[j];
-        const float2 gv = fetch(gi);
-        vx += w * gv.x;
-        vy += w * gv.y;
-        const float wb = cx * gv.x + cy * gv.y;   // W-bar of this stencil point
-        rxb -= wb * dwx[i] * wy[j];
-        ryb -= wb * wx[i] * dwy[j];
-      }
-  };
-  auto gather = [&](const float2 *grid, bool scatter_rule, float cx, float cy, float &vx, float &vy) {
-    gather_with([&](int gi) { return grid[gi]; }, scatter_rule, cx, cy, vx, vy);
-  };
-  // dst = spread of (vx, vy) with the given index rule
-  auto spread_to = [&](float2 *dst, bool scatter_rule, float vx, float vy) {
-    for (int k = tid; k < SS; k += nt) dst[k] = make_float2(0.f, 0.f);
-    if (binned) rforce[my_pos] = make_float2(vx, vy);
-    __syncthreads();
-    bin_spread<1>(dst, rec, M, bend, actl, n_act, S);
-    if (s_irr) {
-      __syncthreads();
-      if (act && !regular) {
-        float *dflat = reinterpret_cast<float *>(dst);
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < 4; ++i) {
+        if (i < l.rows) {
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
             const int p = base + i * S + j;
-            const int gi = scatter_rule ? scatter_index32(p, SS) : gather_index32(p, SS);
+            const int gi = inside ? p : (scatter_rule ? scatter_index32(p, SS) : gather_index32(p, SS));
+            if (gi < 0) continue;
+            const float w = wr[i] * wy[j];
+            const float2 gv = fetch(gi);
+            vx += w * gv.x;
+            vy += w * gv.y;
+            const float wb = cx * gv.x + cy * gv.y;   // W-bar of this stencil point
+            rxb -= wb * dwr[i] * wy[j];
+            ryb -= wb * wr[i] * dwy[j];
+          }
+        }
+      }
+    }
+    pair_sum(vx, vy);
+  };
+  auto from_acc = [&](int gi) { return acc_get(acc, SS, gi); };
+  // acc = spread of (vx, vy) with the given index rule (only marked cells are nonzero)
+  auto spread_acc = [&](bool scatter_rule, float vx, float vy) {
+    unsigned long long *g = reinterpret_cast<unsigned long long *>(acc);
+    int base = base0;
+    asm volatile("" : "+v"(base));
+    for (int idx = tid; idx < n_list; idx += nt) {
+      const int k = list[idx];
+      g[k] = 0ull;
+      g[SS + k] = 0ull;
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        if (i < l.rows) {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const int p = base + i * S + j;
+            const int gi = inside ? p : (scatter_rule ? scatter_index32(p, SS) : gather_index32(p, SS));
             if (gi >= 0) {
-              const float w = wx[i] * wy[j];
-              atomicAdd(&dflat[2 * gi], vx * w);
-              atomicAdd(&dflat[2 * gi + 1], vy * w);
+              const float w = wr[i] * wy[j];
+              acc_add(acc, SS, gi, vx * w, vy * w);
             }
           }
+        }
       }
     }
     __syncthreads();
   };
-  auto spread = [&](bool scatter_rule, float vx, float vy) { spread_to(G, scatter_rule, vx, vy); };
+
   // ---- forward replay of the MDF loop (ib/mdf.py:45-57)
   float ubx, uby, dFx[NIT], dFy[NIT], Fx = 0.f, Fy = 0.f;
-  gather(uw, false, 0.f, 0.f, ubx, uby);   // no W-bar term yet (coefficients 0)
+  gather_with([&](int gi) { return uw[gi]; }, false, 0.f, 0.f, ubx, uby);
 #pragma unroll
   for (int k = 0; k < NIT; ++k) {
     dFx[k] = (Ux - ubx) * ds2;
     dFy[k] = (Uy - uby) * ds2;
     Fx += dFx[k];
     Fy += dFy[k];
-    float2 *A = KEEP ? Ak + SS * k : G;
-    spread_to(A, true, dFx[k], dFy[k]);
-    if (act) {
-      float ax = 0.f, ay = 0.f;
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const float2 av = A[gather_index32(base + i * S + j, SS)];
-          const float w = wx[i] * wy[j];
-          ax += w * (av.x * 0.5f);
-          ay += w * (av.y * 0.5f);
-        }
-      ubx += ax;
-      uby += ay;
+    spread_acc(true, dFx[k], dFy[k]);
+    if constexpr (KEEP) {
+      float2 *A = Ak + SS * k;
+      for (int idx = tid; idx < n_list; idx += nt) {
+        const int c = list[idx];
+        A[c] = acc_get(acc, SS, c);
+      }
     }
+    float ax = 0.f, ay = 0.f;
+    if (act) {
+      int base = base0;
+      asm volatile("" : "+v"(base));
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        if (i < l.rows) {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const float2 av = acc_get(acc, SS, gather_index32(base + i * S + j, SS));
+            const float w = wr[i] * wy[j];
+            ax += w * (av.x * 0.5f);
+            ay += w * (av.y * 0.5f);
+          }
+        }
+      }
+    }
+    pair_sum(ax, ay);
+    ubx += ax;
+    uby += ay;
     __syncthreads();
   }
   // ---- reverse
@@ -463,17 +479,20 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
 #pragma unroll
   for (int k = NIT - 1; k >= 0; --k) {
     // ub_{k+1} = ub_k + W (0.5 A_k), A_k = W^T dF_k
-    const float2 *A = G;
-    if constexpr (KEEP) A = Ak + SS * k;
-    else spread(true, dFx[k], dFy[k]);
     {
       float vx, vy;
-      gather(A, false, 0.5f * ubbx, 0.5f * ubby, vx, vy);
+      if constexpr (KEEP) {
+        const float2 *A = Ak + SS * k;
+        gather_with([&](int gi) { return A[gi]; }, false, 0.5f * ubbx, 0.5f * ubby, vx, vy);
+      } else {
+        spread_acc(true, dFx[k], dFy[k]);
+        gather_with(from_acc, false, 0.5f * ubbx, 0.5f * ubby, vx, vy);
+      }
     }
     __syncthreads();
-    spread(false, 0.5f * ubbx, 0.5f * ubby);   // Abar_k = 0.5 W^T ubbar_{k+1} (gather rule)
+    spread_acc(false, 0.5f * ubbx, 0.5f * ubby);   // Abar_k = 0.5 W^T ubbar_{k+1} (gather rule)
     float abx, aby;
-    gather(G, true, dFx[k], dFy[k], abx, aby);  // dFbar += W Abar_k (scatter rule)
+    gather_with(from_acc, true, dFx[k], dFy[k], abx, aby);   // dFbar += W Abar_k (scatter rule)
     __syncthreads();
     const float dFbx = Fbx + abx, dFby = Fby + aby;
     ubbx -= ds2 * dFbx;
@@ -484,12 +503,13 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
   // ub_0 = W u: W-bar term and the window-velocity adjoint ubar = W^T ubbar_0 (gather rule)
   {
     float vx, vy;
-    gather(uw, false, ubbx, ubby, vx, vy);
+    gather_with([&](int gi) { return uw[gi]; }, false, ubbx, ubby, vx, vy);
   }
-  spread(false, ubbx, ubby);
-  // fbar_t += (c_i - u) . ubar / rho on the window cells
-  for (int k = tid; k < SS; k += nt) {
-    const float2 ub0 = G[k];
+  spread_acc(false, ubbx, ubby);
+  // fbar_t += (c_i - u) . ubar / rho on the window cells the stencils reach
+  for (int idx = tid; idx < n_list; idx += nt) {
+    const int k = list[idx];
+    const float2 ub0 = acc_get(acc, SS, k);
     if (ub0.x == 0.f && ub0.y == 0.f) continue;
     const size_t cell = (size_t)(xc + k / S) * P.ny + (yc + k % S);
     float rho = 0.f;
@@ -500,12 +520,16 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
     for (int i = 0; i < 9; ++i)
       fbar[i * n + cell] += (ub0.x * ((float)D2Q9::cx[i] - u.x) + ub0.y * ((float)D2Q9::cy[i] - u.y)) / rho;
   }
-  // marker-position adjoints: delta (radial direction), displacement
-  if (act) grad_delta[m] += (double)rxb * dir_x[m] + (double)ryb * dir_y[m];
+  // marker-position adjoints: delta (radial direction), displacement; the
+  // marker-level values V are held by both threads of a marker, counted once
+  const bool lead = act && l.first_row() == 0;
+  float rxm = rxb, rym = ryb;
+  pair_sum(rxm, rym);
+  if (lead) grad_delta[m] += (double)rxm * dir_x[m] + (double)rym * dir_y[m];
   const double srx = block_sum_d(act ? (double)rxb : 0.0, red);
   const double sry = block_sum_d(act ? (double)ryb : 0.0, red);
-  const double svx = block_sum_d(act ? (double)Vbx : 0.0, red);
-  const double svy = block_sum_d(act ? (double)Vby : 0.0, red);
+  const double svx = block_sum_d(lead ? (double)Vbx : 0.0, red);
+  const double svy = block_sum_d(lead ? (double)Vby : 0.0, red);
   if (tid == 0) {
     adj_body[0] = s_ab[0];
     adj_body[1] = s_ab[1];

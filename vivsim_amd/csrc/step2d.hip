@@ -4,15 +4,21 @@
 
 namespace vsx {
 
-// Stand-alone IB kernel (the first step of a run, and the non-fused path).
+// Stand-alone IB kernel (the first step of a run, and the non-fused path):
+// the fixed-point LDS spread of the fused steps for up to one marker per
+// thread, the binned spread for more markers (MPT > 1).
 template <class SRC, bool PULL, int MPT>
 __global__ __launch_bounds__(kIbThreads) void k_viv_ib(StepCtx c_arg, VsxIB2D ib,
                                                         const typename SRC::T *__restrict__ src, int step) {
   const StepCtx &c = kernel_ctx();
   extern __shared__ float lds[];
   const int gstep = ib.step0 + step, slot = gstep & 1;
-  ib_block<SRC, PULL, MPT, 4, 0, false, 4>(c, ib, src, step, gstep, 0, ib.body + 6 * slot, ib.body + 6 * (slot ^ 1),
-                                        lds);
+  if constexpr (MPT == 1)
+    ib_atomic<SRC, PULL ? kWinPull : kWinLocal, 2, 3, true>(c, ib, src, nullptr, step, gstep, nullptr,
+                                                            ib.body + 6 * slot, ib.body + 6 * (slot ^ 1), lds);
+  else
+    ib_block<SRC, PULL, MPT, 4, 0, false, 4>(c, ib, src, step, gstep, 0, ib.body + 6 * slot,
+                                          ib.body + 6 * (slot ^ 1), lds);
   publish_ready(ib, gstep + 1);
 }
 
@@ -20,12 +26,16 @@ __global__ __launch_bounds__(kIbThreads) void k_viv_ib(StepCtx c_arg, VsxIB2D ib
 
 template <class SRC, bool PULL>
 int launch_viv_ib(const StepCtx &c, const VsxIB2D &ib, const void *src, int step, hipStream_t s) {
-  const size_t lds = 4 * (size_t)ib_lds_words(ib.win, ib.n_markers, ib.n_markers <= kIbThreads);
   auto S = static_cast<const typename SRC::T *>(src);
-  if (ib.n_markers <= kIbThreads)
-    hipLaunchKernelGGL((k_viv_ib<SRC, PULL, 1>), dim3(1), dim3(kIbThreads), lds, s, c, ib, S, step);
-  else
+  int max_lds = 0;
+  if (int rc = max_lds_per_block(&max_lds)) return rc;
+  const size_t lds_atomic = 4 * (size_t)ib_split_lds_words(ib.win);
+  if (ib.n_markers <= kIbThreads && lds_atomic + 2048 <= (size_t)max_lds) {
+    hipLaunchKernelGGL((k_viv_ib<SRC, PULL, 1>), dim3(1), dim3(kIbThreads), lds_atomic, s, c, ib, S, step);
+  } else {
+    const size_t lds = 4 * (size_t)ib_lds_words(ib.win, ib.n_markers, false);
     hipLaunchKernelGGL((k_viv_ib<SRC, PULL, kMarkersPerThread>), dim3(1), dim3(kIbThreads), lds, s, c, ib, S, step);
+  }
   VSX_CHECK_LAUNCH();
   return VSX_OK;
 }
