@@ -199,6 +199,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=2000)
+    ap.add_argument("--form", default="default", choices=["default", "one_launch", "separate"],
+                    help="launch form of the headline run (default: the VIV2D default, the pipelined step); "
+                         "byte-counter passes use one_launch, whose kernels never wait for one another")
     ap.add_argument("--only", choices=["grid_4096", "grid_4096_f16s", "grid_4096_e3m12s", "sphere3d", "batch64",
                                        "adjoint"],
                     help="profiling aid: run only this variant (no headline line) and print its result")
@@ -234,7 +237,7 @@ def main():
         print(json.dumps({args.only: fns[args.only]()}), flush=True)
         return 0
 
-    main_run = run_gpu(args, args.storage, args.steps, args.warmup)
+    main_run = run_gpu(args, args.storage, args.steps, args.warmup, args.form)
     print(f"headline: {main_run['glups']:.3f} GLUPS, {main_run['ms_per_step'] * 1e3:.2f} us / step, "
           f"bulk kernel {main_run['kernel_ms'] * 1e3:.2f} us", file=sys.stderr, flush=True)
     variants = {}
@@ -301,8 +304,10 @@ def main():
                      # HBM bytes per launch come from rocprofv3 FETCH_SIZE / WRITE_SIZE passes, which cannot
                      # run inside this timed process; the measured figures are kept under profiles/
                      "traffic": None,
-                     "kernel": "k_viv_fused (pull-stream + walls + MRT/Guo bulk pass; pipelined form: the IB "
-                               "kernel of the next step runs on a second stream)",
+                     "kernel": {2: "k_viv_fused (pull-stream + walls + MRT/Guo bulk pass; pipelined form: the IB "
+                                   "kernel of the next step runs on a second stream)",
+                                1: "k_viv_fused (one launch per step: IB workgroup + pull-stream/MRT/Guo tiles)",
+                                0: "k_bulk (bulk pass of the separate IB and bulk launches)"}[main_run["mode"]],
                      "kernel_us": round(main_run["kernel_ms"] * 1e3, 3),
                      "bytes_per_launch": main_run["bytes_per_launch"], "launches": main_run["kernel_launches"]},
         "variants": variants,
