@@ -52,12 +52,14 @@ def run_gpu(args, storage, steps, warmup, mode="default"):
     solver = sim.VIV2D(case, storage=storage, **kw)
     solver.set_body(v=(0.0, 1e-3 * case.U0))
     u = torch.as_tensor(case.initial_u(), dtype=torch.float32, device="cuda")
-    f = lbm.get_equilibrium(torch.ones(case.nx, case.ny, device="cuda"), u)
-    f, _, _ = solver.run(f, warmup, history=False)
-    # caller-owned output and histories, as a loop of runs would keep them
-    out = torch.empty_like(f)
-    hist = (torch.empty((steps, 2), dtype=torch.float64, device="cuda"),
-            torch.empty((steps, 2), dtype=torch.float64, device="cuda"))
+    f0 = lbm.get_equilibrium(torch.ones(case.nx, case.ny, device="cuda"), u)
+    # caller-owned fields and histories, as a loop of runs keeps them: the warm-up
+    # run writes the buffer the timed run reads and fills the same histories
+    f = torch.empty_like(f0)
+    hist = (torch.empty((max(steps, warmup), 2), dtype=torch.float64, device="cuda"),
+            torch.empty((max(steps, warmup), 2), dtype=torch.float64, device="cuda"))
+    f, _, _ = solver.run(f0, warmup, out=f, history=hist)
+    out = f0
     torch.cuda.synchronize()
     # throughput: the plain run, no instrumentation in the stream
     t0 = time.perf_counter()
@@ -81,7 +83,7 @@ def run_gpu(args, storage, steps, warmup, mode="default"):
     return dict(glups=cells * steps / sec / 1e9, ms_per_step=sec / steps * 1e3, kernel_ms=mean_ms,
                 kernel_launches=int(kt.size), bytes_per_launch=bpc * cells,
                 achieved_gbs=bpc * cells / (mean_ms * 1e-3) / 1e9, mode=solver.ib.fused,
-                lift_last=float(hh[-1, 1].item()), d_last=float(hd[-1, 1].item()))
+                lift_last=float(hh[steps - 1, 1].item()), d_last=float(hd[steps - 1, 1].item()))
 
 
 def timed(fn):
