@@ -237,9 +237,9 @@ __host__ __device__ inline AdjLds adj_lds_layout(int S, int M, int nit, bool kee
 }
 
 // Reverse IB step.  Thread t takes marker mk and stencil rows [r0, r0 + rows)
-// as in the forward IB workgroup (two threads per marker when 2 M <= 1024);
-// only the window cells the stencils read are pulled, zeroed and converted;
-// every spread adds int64 fixed-point values in LDS (exact, independent of
+// as in the forward IB workgroup (four threads per marker when 4 M <= 1024,
+// two when 2 M <= 1024); only the window cells the stencils read are pulled,
+// zeroed and converted; every spread adds int64 fixed-point values in LDS (exact, independent of
 // the order of the additions), with the index rule of the operation for
 // stencils that leave the window.  KEEP: the forward replay's spread grids
 // A_k stay in LDS for the reverse loop, else they are spread again.
@@ -265,7 +265,6 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
   const IbLane l = ib_lane(M);
   const bool act = l.mk < M;
   const int m = l.mk;
-  const int partner = (int)(threadIdx.x & 63) ^ 1;
 
   // round trip 1: body state, its adjoint, marker inputs
   if (tid < 6) {
@@ -364,10 +363,8 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
   const float Ux = (float)s_body[2], Uy = (float)s_body[3];
   float rxb = 0.f, ryb = 0.f;   // this thread's part of the marker-position adjoint
   auto pair_sum = [&](float &vx, float &vy) {
-    if (l.tpm == 2) {
-      vx += lane_get(vx, partner);
-      vy += lane_get(vy, partner);
-    }
+    vx = marker_sum(vx, l);
+    vy = marker_sum(vy, l);
   };
   // gather of a 2-component grid at this thread's rows: value sums (summed over
   // the marker's threads) and the W-bar terms with coefficients (cx, cy)
@@ -521,7 +518,7 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
       fbar[i * n + cell] += (ub0.x * ((float)D2Q9::cx[i] - u.x) + ub0.y * ((float)D2Q9::cy[i] - u.y)) / rho;
   }
   // marker-position adjoints: delta (radial direction), displacement; the
-  // marker-level values V are held by both threads of a marker, counted once
+  // marker-level values V are held by every thread of a marker, counted once
   const bool lead = act && l.first_row() == 0;
   float rxm = rxb, rym = ryb;
   pair_sum(rxm, rym);

@@ -31,6 +31,15 @@ __device__ __forceinline__ int lane_get(int v, int src) {
   return __builtin_amdgcn_ds_bpermute(src << 2, v);
 }
 
+// Exchange inside each group of four lanes (DPP quad_perm, no LDS round trip):
+// lane 4q + i reads lane 4q + ((CTRL >> 2i) & 3).
+template <int CTRL>
+__device__ __forceinline__ float quad_get(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+constexpr int kQuadXor1 = 0xB1;   // [1, 0, 3, 2]
+constexpr int kQuadXor2 = 0x4E;   // [2, 3, 0, 1]
+
 // Window origin of step t from the body state before the step:
 // int32(IB_X0 + d) truncated (examples/2d/vortex_induced_vibration.py:104-105),
 // and that origin clamped to the grid as lax.dynamic_slice clamps it.
@@ -922,9 +931,10 @@ __host__ __device__ inline int ib_atomic_lds_words(int S) { return 4 * S * S; }
 __host__ __device__ inline int ib_split_lds_words(int S) { return 6 * S * S + (S * S + 3) / 4 + (S * S + 1) / 2; }
 
 // Thread t of the IB workgroup works on marker mk and stencil rows
-// [r0, r0 + rows): with M <= nt / 2 two threads share a marker (rows 0-1 and
+// [r0, r0 + rows): with M <= nt / 4 four threads share a marker (one row
+// each, a group of four lanes), with M <= nt / 2 two threads (rows 0-1 and
 // 2-3, adjacent lanes), else one thread takes all four rows.  Consecutive
-// lane pairs hold markers nwave apart along the body, so the lanes of one
+// lane groups hold markers nwave apart along the body, so the lanes of one
 // atomic mostly address different cells.
 struct IbLane {
   int mk, r0, rows, tpm;
@@ -933,14 +943,26 @@ struct IbLane {
   __device__ __forceinline__ int first_row() const {
     int t = threadIdx.x;
     asm volatile("" : "+v"(t));
-    return tpm == 2 ? 2 * (t & 1) : 0;
+    return tpm == 4 ? (t & 3) : tpm == 2 ? 2 * (t & 1) : 0;
   }
 };
+// Sum of a per-thread partial over the threads of the marker; every thread of
+// the marker gets the same value ((r0 + r1) + (r2 + r3) with four threads)
+__device__ __forceinline__ float marker_sum(float v, const IbLane &l) {
+  if (l.tpm >= 2) v += quad_get<kQuadXor1>(v);
+  if (l.tpm == 4) v += quad_get<kQuadXor2>(v);
+  return v;
+}
 __device__ __forceinline__ IbLane ib_lane(int M) {
   const int tid = threadIdx.x, nt = blockDim.x, nwave = nt >> 6, lane = tid & 63, wave = tid >> 6;
   IbLane l;
-  l.tpm = __builtin_amdgcn_readfirstlane(2 * M <= nt ? 2 : 1);   // uniform: kept in a scalar register
-  if (l.tpm == 2) {
+  // uniform: kept in a scalar register
+  l.tpm = __builtin_amdgcn_readfirstlane(4 * M <= nt ? 4 : 2 * M <= nt ? 2 : 1);
+  if (l.tpm == 4) {
+    l.mk = (lane >> 2) * nwave + wave;
+    l.r0 = lane & 3;
+    l.rows = 1;
+  } else if (l.tpm == 2) {
     l.mk = (lane >> 1) * nwave + wave;
     l.r0 = 2 * (lane & 1);
     l.rows = 2;
@@ -961,6 +983,36 @@ __device__ __forceinline__ float2 interp_rows(const float *grid, int base, int S
                                               const float (&wy)[4]) {
   const int SS = S * S;
   asm volatile("" : "+v"(base));   // indices recomputed at every call (register budget)
+  auto fetch = [&](int p) {
+    const int k = INSIDE ? p : gather_index32(p, SS);
+    return ACC ? acc_get(grid, SS, k) : reinterpret_cast<const float2 *>(grid)[k];
+  };
+  if (l.tpm == 4) {
+    // the summation order of two threads per marker, (row 0 then row 1) +
+    // (row 2 then row 3), so that both mappings give the same bits: rows 1
+    // and 3 continue from the sums of rows 0 and 2, taken from the lane before
+    float2 v[4];
+    float w[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      v[j] = fetch(base + j);
+      w[j] = wr[0] * wy[j];
+    }
+    float sx = 0.f, sy = 0.f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      sx += w[j] * v[j].x;
+      sy += w[j] * v[j].y;
+    }
+    float tx = quad_get<0xA0>(sx), ty = quad_get<0xA0>(sy);   // [0, 0, 2, 2]
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      tx += w[j] * v[j].x;
+      ty += w[j] * v[j].y;
+    }
+    // lanes 1 and 3 of the group hold the two pair sums
+    return make_float2(quad_get<0x55>(tx) + quad_get<0xFF>(tx), quad_get<0x55>(ty) + quad_get<0xFF>(ty));
+  }
   float sx = 0.f, sy = 0.f;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
@@ -968,8 +1020,7 @@ __device__ __forceinline__ float2 interp_rows(const float *grid, int base, int S
       const int rb = base + i * S;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int k = INSIDE ? rb + j : gather_index32(rb + j, SS);
-        const float2 v = ACC ? acc_get(grid, SS, k) : reinterpret_cast<const float2 *>(grid)[k];
+        const float2 v = fetch(rb + j);
         const float w = wr[i] * wy[j];
         sx += w * v.x;
         sy += w * v.y;
@@ -977,12 +1028,7 @@ __device__ __forceinline__ float2 interp_rows(const float *grid, int base, int S
       asm volatile("" : "+v"(sx), "+v"(sy));
     }
   }
-  if (l.tpm == 2) {
-    const int p = (int)(threadIdx.x & 63) ^ 1;
-    sx += lane_get(sx, p);
-    sy += lane_get(sy, p);
-  }
-  return make_float2(sx, sy);
+  return make_float2(marker_sum(sx, l), marker_sum(sy, l));
 }
 template <bool ACC>
 __device__ __forceinline__ float2 interp_rows(const float *grid, int base, int S, const IbLane &l, const float (&wr)[4],
@@ -1029,10 +1075,10 @@ __device__ __forceinline__ float peskin4(float r) {
 }
 
 // Stencil weights of this thread's rows (wr: x nodes r0 .. r0 + rows - 1,
-// zero beyond) and of the marker's four y nodes; with two threads per marker
-// each computes two of the y weights and takes the other two from its
-// partner lane.  Nodes ref - 1 .. ref + 2, ref = floor(x), weights phi(node - x)
-// in float32 (ib/stencil.py:7-51).
+// zero beyond) and of the marker's four y nodes; with two or four threads per
+// marker each computes its share of the y weights and takes the others from
+// the marker's other lanes.  Nodes ref - 1 .. ref + 2, ref = floor(x), weights
+// phi(node - x) in float32 (ib/stencil.py:7-51).
 template <class PHI>
 __device__ __forceinline__ void ib_weights(PHI phi, const IbLane &l, bool live, float rx, float ry, float (&wr)[4],
                                            float (&wy)[4]) {
@@ -1040,10 +1086,16 @@ __device__ __forceinline__ void ib_weights(PHI phi, const IbLane &l, bool live, 
   const int xref = (int)floorf(rx), yref = (int)floorf(ry);
 #pragma unroll
   for (int i = 0; i < 4; ++i) wr[i] = (live && i < l.rows) ? phi((float)(xref - 1 + r0 + i) - rx) : 0.f;
-  if (l.tpm == 2) {
+  if (l.tpm == 4) {
+    // y weight r0 here; lane 4q + j of the marker holds weight j
+    const float a = phi((float)(yref - 1 + r0) - ry);
+    wy[0] = quad_get<0x00>(a);
+    wy[1] = quad_get<0x55>(a);
+    wy[2] = quad_get<0xAA>(a);
+    wy[3] = quad_get<0xFF>(a);
+  } else if (l.tpm == 2) {
     const float a = phi((float)(yref - 1 + r0) - ry), b = phi((float)(yref + r0) - ry);
-    const int p = (int)(threadIdx.x & 63) ^ 1;
-    const float pa = lane_get(a, p), pb = lane_get(b, p);
+    const float pa = quad_get<kQuadXor1>(a), pb = quad_get<kQuadXor1>(b);
     wy[0] = r0 == 0 ? a : pa;
     wy[1] = r0 == 0 ? b : pb;
     wy[2] = r0 == 0 ? pa : a;
