@@ -207,6 +207,9 @@ typedef struct VsxAdj2D {
   double *u_in_part;        /* device [T]: per-step inlet-velocity gradient terms                    */
   double *grad_delta;       /* device [M] output: dJ / d delta_m                                    */
   double *out;              /* device [3] output: J, dJ / d stiffness, dJ / d u_in                    */
+  int32_t *sync;            /* device [4] scratch of the reverse pass (hand-off counters between its
+                               bulk and IB kernels); sync[3] != 0 after the call: a bounded wait
+                               timed out and the gradient is invalid                                */
 } VsxAdj2D;
 
 /* ------------------------------------------------------------------ library */

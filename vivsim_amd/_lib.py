@@ -71,7 +71,7 @@ class VsxAdj2D(ctypes.Structure):
                 ("ckpt", c_void_p), ("ckpt_body", c_void_p), ("tape_f", c_void_p), ("tape_g", c_void_p),
                 ("tape_body", c_void_p), ("adj_a", c_void_p), ("adj_b", c_void_p), ("grid", c_void_p),
                 ("adj_body", c_void_p), ("hist_d", c_void_p), ("u_in_part", c_void_p), ("grad_delta", c_void_p),
-                ("out", c_void_p)]
+                ("out", c_void_p), ("sync", c_void_p)]
 
 
 class VsxIB3D(ctypes.Structure):

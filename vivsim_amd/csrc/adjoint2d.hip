@@ -12,11 +12,15 @@
 //   g_t, H_t = MDF(u(f_t), W(x_m + d_t)), body_{t+1} = Newmark(body_t, H_t)
 // Reverse step t, given fbar_{t+1} and the adjoint of body_{t+1}:
 //   k_adj_walls  wall adjoint in place on fbar_{t+1} (-> sbar), inlet-velocity term
+//                (inside k_adj_ib of step t + 1 when that step's window touches a
+//                wall column)
 //   k_adj_bulk   pbar_i(x) = sbar_i(x + c_i); per-cell transpose of the collision +
 //                forcing Jacobian -> fbar_t, and gbar_t on the force window
 //   k_adj_ib     Newmark adjoint, replay + reverse of the MDF loop (transposes of
 //                interpolate / spread, kernel derivative phi' for the marker
-//                positions), window-velocity adjoint added to fbar_t
+//                positions), window-velocity adjoint added to fbar_t.  It runs
+//                on its own stream beside k_adj_bulk of step t (replay first,
+//                then a wait for the bulk kernel's output), see viv_adjoint.
 #include <algorithm>
 #include <vector>
 
@@ -39,6 +43,23 @@ __device__ __forceinline__ double block_sum_d(double v, double *red) {
   for (int w = 0; w < (int)((blockDim.x + 63) >> 6); ++w) s += red[w];
   return s;
 }
+// Four block sums at once (red: 4 x 16 doubles); v holds the sums on return.
+__device__ __forceinline__ void block_sum4_d(double (&v)[4], double *red) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) v[k] = wave_sum(v[k]);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) red[k * 16 + (threadIdx.x >> 6)] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    double s = 0.0;
+    for (int w = 0; w < (int)((blockDim.x + 63) >> 6); ++w) s += red[k * 16 + w];
+    v[k] = s;
+  }
+}
 
 // feq_i(rho, u) = rho w_i (1 + 3 c.u + 4.5 (c.u)^2 - 1.5 |u|^2)   (lbm/basic.py:114-135)
 __device__ __forceinline__ void equilibrium_raw(float rho, float ux, float uy, float (&feq)[9]) {
@@ -57,11 +78,42 @@ __device__ __forceinline__ float dfeq_du(int i, int a, float rho, float cu, floa
 }
 
 // ------------------------------------------------------------------ walls
-// One workgroup; y loop.  adj holds fbar_{t+1} and becomes sbar (adjoint of
-// the streamed field before the walls).  fn = f_{t+1}.
-__global__ __launch_bounds__(1024) void k_adj_walls(float *adj, const float *__restrict__ fn, int nx, int ny,
-                                                    float u_in, double *u_in_part) {
-  __shared__ double red[16];
+// Hand-off words of the reverse pass (VsxAdj2D.sync): [0] count of published
+// bulk workgroups, [1] the count the IB kernels have consumed, [2] number of
+// finished IB kernels, [3] error code (1: a bounded wait timed out).
+constexpr long kAdjSpinLimit = 1l << 22;
+__device__ __forceinline__ void adj_wait(int32_t *sync, int word, int target) {
+  long spins = 0;
+  while (__hip_atomic_load(&sync[word], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    __builtin_amdgcn_s_sleep(2);
+    if (++spins > kAdjSpinLimit) {
+      __hip_atomic_store(&sync[3], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      break;
+    }
+  }
+}
+
+// Clamped window column of a body state (examples/2d/vortex_induced_vibration.py:104-105)
+__device__ __forceinline__ int adj_win_col(double x0, const double *body, int nx, int S) {
+  return min(max((int)trunc(x0 + body[4]), 0), nx - S);
+}
+// The window (columns xc .. xc + S - 1) touches none of the columns the wall
+// adjoint writes (0, nx - 2, nx - 1): the walls of step t - 1 and the IB
+// adjoint of step t then write disjoint cells and may run in either order.
+__device__ __forceinline__ bool adj_walls_clear(int xc, int S, int nx) { return xc >= 1 && xc + S <= nx - 2; }
+
+// Wall adjoint (rows y = threadIdx.x + k blockDim.x) in place on adj (= fbar_{t+1},
+// becomes sbar, the adjoint of the streamed field before the walls); fn = f_{t+1}.
+// AGENT: adj is read and written with agent-scope accesses (a hand-off between
+// concurrent kernels).  Returns this thread's part of the inlet-velocity term.
+template <bool AGENT>
+__device__ __forceinline__ double adj_walls_rows(float *__restrict__ adj, const float *__restrict__ fn, int nx, int ny,
+                                                 float u_in) {
+  auto ld = [](const float *q) { return AGENT ? ld_agent(q) : *q; };
+  auto st = [](float *q, float v) {
+    if (AGENT) st_agent(q, v);
+    else *q = v;
+  };
   const size_t n = (size_t)nx * ny;
   double part = 0.0;
   for (int y = threadIdx.x; y < ny; y += blockDim.x) {
@@ -71,9 +123,9 @@ __global__ __launch_bounds__(1024) void k_adj_walls(float *adj, const float *__r
       float s = 0.f;
 #pragma unroll
       for (int i = 0; i < 9; ++i) {
-        const float q = adj[i * n + c0];
+        const float q = ld(&adj[i * n + c0]);
         s += q * dfeq_du(i, 0, 1.f, (float)D2Q9::cx[i] * u_in, u_in);
-        adj[i * n + c0] = 0.f;
+        st(&adj[i * n + c0], 0.f);
       }
       part += (double)s;
     }
@@ -83,7 +135,7 @@ __global__ __launch_bounds__(1024) void k_adj_walls(float *adj, const float *__r
       float q[9], fw[9], sb[9];
 #pragma unroll
       for (int i = 0; i < 9; ++i) {
-        q[i] = adj[i * n + cw];
+        q[i] = ld(&adj[i * n + cw]);
         fw[i] = fn[i * n + cw];
         sb[i] = fn[i * n + cb];
       }
@@ -115,13 +167,27 @@ __global__ __launch_bounds__(1024) void k_adj_walls(float *adj, const float *__r
       for (int j = 0; j < 9; ++j) {
         // d u_n / d s_j = -sign / rho_w * coef_j (right wall: sign = -1): 1 for c_x = 0, 2 for c_x > 0
         const float coef = D2Q9::cx[j] == 0 ? 1.f : (D2Q9::cx[j] > 0 ? 2.f : 0.f);
-        adj[j * n + cw] = gx * coef;
+        st(&adj[j * n + cw], gx * coef);
         const float cxj = (float)D2Q9::cx[j], cyj = (float)D2Q9::cy[j];
         const float extra = q[j] + gy * (cyj - uby) / rb + rhobar + (uxbar * (cxj - ubx) + uybar * (cyj - uby)) / rb;
-        adj[j * n + cb] += extra;
+        st(&adj[j * n + cb], ld(&adj[j * n + cb]) + extra);
       }
     }
   }
+  return part;
+}
+
+// Wall adjoint of step t, one workgroup, on the bulk kernels' stream after the
+// bulk kernel of step t + 1.  body1 (the body state of step t + 1, NULL for the
+// last step of the run): when the IB window of step t + 1 touches a wall
+// column, the walls run inside that step's IB kernel instead (after its
+// window update), and this kernel does nothing.
+__global__ __launch_bounds__(1024) void k_adj_walls(float *adj, const float *__restrict__ fn, int nx, int ny,
+                                                    float u_in, double *u_in_part, const double *body1, double x0,
+                                                    int S) {
+  __shared__ double red[16];
+  if (body1 && !adj_walls_clear(adj_win_col(x0, body1, nx, S), S, nx)) return;
+  const double part = adj_walls_rows<false>(adj, fn, nx, ny, u_in);
   const double tot = block_sum_d(part, red);
   if (threadIdx.x == 0) *u_in_part = tot;
 }
@@ -131,27 +197,55 @@ struct AdjBulk {
   int nx, ny, S;
   float rates[9];
   double win_x0, win_y0;
+  int32_t *sync;   // hand-off words (adj_wait)
 };
 
+// Reverse bulk step t.  It runs while the IB kernel of step t + 1 may still
+// be working on sbar: the workgroups whose pull reads a column that kernel
+// writes (its window +- 1; the wall columns when it runs the walls) wait for
+// it to finish (sync[2] >= need_ib) and read sbar with agent-scope loads.
+// The workgroups whose output the IB kernel of step t reads (its window
+// columns; the wall columns when it runs the walls) store with agent-scope
+// stores and count themselves in sync[0].
 __global__ __launch_bounds__(256) void k_adj_bulk(AdjBulk P, const float *__restrict__ sbar, float *__restrict__ fbar,
                                                   const float *__restrict__ ft, const float *__restrict__ gwin,
-                                                  const double *__restrict__ body, float *__restrict__ gbar) {
+                                                  const double *__restrict__ body, const double *__restrict__ body1,
+                                                  float *__restrict__ gbar, int need_ib) {
   const int y = blockIdx.x * blockDim.x + threadIdx.x, x = blockIdx.y;
-  if (y >= P.ny) return;
-  const size_t n = (size_t)P.nx * P.ny, cell = (size_t)x * P.ny + y;
+  const bool valid = y < P.ny;
+  const size_t n = (size_t)P.nx * P.ny, cell = (size_t)x * P.ny + (valid ? y : 0);
   const int S = P.S;
-  const int xc = min(max((int)trunc(P.win_x0 + body[4]), 0), P.nx - S);
+  const int xc = adj_win_col(P.win_x0, body, P.nx, S), xc1 = adj_win_col(P.win_x0, body1, P.nx, S);
   const int yc = min(max((int)trunc(P.win_y0 + body[5]), 0), P.ny - S);
+  // (the band of this step's own window too: its gbar stores must not
+  // overtake the reads of the IB kernel of step t + 1)
+  const bool walls_col = x == 0 || x >= P.nx - 2;
+  const bool waiter = (x >= xc1 - 1 && x <= xc1 + S) || (x >= xc - 1 && x <= xc + S) ||
+                      (!adj_walls_clear(xc1, S, P.nx) && (x <= 1 || x >= P.nx - 3));
+  const bool publish = (x >= xc && x < xc + S) || (!adj_walls_clear(xc, S, P.nx) && walls_col);
+  if (waiter) {
+    if (threadIdx.x == 0) adj_wait(P.sync, 2, need_ib);
+    __syncthreads();
+  }
   const int r0 = x - xc, r1 = y - yc;
-  const bool in_win = r0 >= 0 && r0 < S && r1 >= 0 && r1 < S;
+  const bool in_win = valid && r0 >= 0 && r0 < S && r1 >= 0 && r1 < S;
   float pb[9], f[9];
+  if (valid) {
 #pragma unroll
-  for (int i = 0; i < 9; ++i) {
-    int xs = x + D2Q9::cx[i], ys = y + D2Q9::cy[i];
-    xs = xs < 0 ? xs + P.nx : (xs >= P.nx ? xs - P.nx : xs);
-    ys = ys < 0 ? ys + P.ny : (ys >= P.ny ? ys - P.ny : ys);
-    pb[i] = sbar[i * n + (size_t)xs * P.ny + ys];
-    f[i] = ft[i * n + cell];
+    for (int i = 0; i < 9; ++i) {
+      int xs = x + D2Q9::cx[i], ys = y + D2Q9::cy[i];
+      xs = xs < 0 ? xs + P.nx : (xs >= P.nx ? xs - P.nx : xs);
+      ys = ys < 0 ? ys + P.ny : (ys >= P.ny ? ys - P.ny : ys);
+      const float *src = &sbar[i * n + (size_t)xs * P.ny + ys];
+      pb[i] = waiter ? ld_agent(src) : *src;
+      f[i] = ft[i * n + cell];
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      pb[i] = 0.f;
+      f[i] = D2Q9::w[i];
+    }
   }
   float rho = 0.f, jx = 0.f, jy = 0.f;
 #pragma unroll
@@ -201,12 +295,22 @@ __global__ __launch_bounds__(256) void k_adj_bulk(AdjBulk P, const float *__rest
   gyb += usyb * 0.5f / rho;
   const float jxb = usxb / rho, jyb = usyb / rho;
   rhobar -= (usxb * usx + usyb * usy) / rho;
+  if (valid) {
 #pragma unroll
-  for (int i = 0; i < 9; ++i)
-    fbar[i * n + cell] = pb[i] - q[i] + rhobar + jxb * (float)D2Q9::cx[i] + jyb * (float)D2Q9::cy[i];
+    for (int i = 0; i < 9; ++i) {
+      const float v = pb[i] - q[i] + rhobar + jxb * (float)D2Q9::cx[i] + jyb * (float)D2Q9::cy[i];
+      if (publish) st_agent(&fbar[i * n + cell], v);
+      else fbar[i * n + cell] = v;
+    }
+  }
   if (in_win) {
-    gbar[r0 * S + r1] = gxb;
-    gbar[S * S + r0 * S + r1] = gyb;
+    st_agent(&gbar[r0 * S + r1], gxb);
+    st_agent(&gbar[S * S + r0 * S + r1], gyb);
+  }
+  if (publish) {
+    wait_mem();
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_fetch_add(&P.sync[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -214,6 +318,9 @@ __global__ __launch_bounds__(256) void k_adj_bulk(AdjBulk P, const float *__rest
 struct AdjIB {
   int nx, ny;
   double inv_T;
+  int32_t *sync;   // hand-off words (adj_wait)
+  int nby;         // workgroups per column of the reverse bulk kernel
+  float u_in;      // inlet velocity (wall adjoint)
 };
 
 // LDS of k_adj_ib in 4-byte words (S = window side, SS = S * S):
@@ -248,7 +355,8 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
                                                  const float *__restrict__ gbar, float *__restrict__ fbar,
                                                  const double *__restrict__ body, const double *__restrict__ body1,
                                                  double *__restrict__ adj_body, double *__restrict__ grad_delta,
-                                                 const double *__restrict__ dir_x, const double *__restrict__ dir_y) {
+                                                 const double *__restrict__ dir_x, const double *__restrict__ dir_y,
+                                                 double *__restrict__ u_in_prev, int done_value) {
   extern __shared__ float lds[];
   const int S = ib.win, SS = S * S, M = ib.n_markers;
   const int tid = threadIdx.x, nt = blockDim.x;
@@ -260,7 +368,7 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
   uint8_t *mark = reinterpret_cast<uint8_t *>(lds + lay.mark);
   uint16_t *list = reinterpret_cast<uint16_t *>(lds + lay.list);
   __shared__ double s_body[6], s_ab[6], s_misc[4];
-  __shared__ double red[16];
+  __shared__ double red[16], red4[4 * 16];
   __shared__ int s_org[4], s_scan[kIbThreads / 64 + 1];
   const IbLane l = ib_lane(M);
   const bool act = l.mk < M;
@@ -271,12 +379,16 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
     s_body[tid] = body[tid];
     s_ab[tid] = adj_body[tid];
   }
-  double mxd = 0.0, myd = 0.0;
+  double mxd = 0.0, myd = 0.0, gd0 = 0.0, dxm = 0.0, dym = 0.0;
   float ds2 = 0.f;
   if (act) {
     mxd = ib.marker_x[m];
     myd = ib.marker_y[m];
     ds2 = (ib.ds_arr ? ib.ds_arr[m] : ib.ds) * 2.0f;
+    // the marker's gradient so far and its direction, for the update at the end
+    gd0 = grad_delta[m];
+    dxm = dir_x[m];
+    dym = dir_y[m];
   }
   for (int k = tid; k < (SS + 3) / 4; k += nt) reinterpret_cast<uint32_t *>(mark)[k] = 0u;
   __syncthreads();
@@ -326,9 +438,42 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
         wr[i] = delta_kernel(ib.kernel, ax);
         dwr[i] = delta_kernel_deriv(ib.kernel, ax);
       }
-      const float ay = (float)(refy - 1 + i) - ry;
-      wy[i] = delta_kernel(ib.kernel, ay);
-      dwy[i] = delta_kernel_deriv(ib.kernel, ay);
+    }
+    // the four y nodes: with several threads per marker each evaluates its
+    // share and takes the others from the marker's lanes
+    if (l.tpm == 4) {
+      const float ay = (float)(refy - 1 + r0) - ry;
+      const float a = delta_kernel(ib.kernel, ay), da = delta_kernel_deriv(ib.kernel, ay);
+      wy[0] = quad_get<0x00>(a);
+      wy[1] = quad_get<0x55>(a);
+      wy[2] = quad_get<0xAA>(a);
+      wy[3] = quad_get<0xFF>(a);
+      dwy[0] = quad_get<0x00>(da);
+      dwy[1] = quad_get<0x55>(da);
+      dwy[2] = quad_get<0xAA>(da);
+      dwy[3] = quad_get<0xFF>(da);
+    } else if (l.tpm == 2) {
+      const float ay0 = (float)(refy - 1 + r0) - ry, ay1 = (float)(refy + r0) - ry;
+      const float a0 = delta_kernel(ib.kernel, ay0), a1 = delta_kernel(ib.kernel, ay1);
+      const float d0 = delta_kernel_deriv(ib.kernel, ay0), d1 = delta_kernel_deriv(ib.kernel, ay1);
+      const float pa0 = quad_get<kQuadXor1>(a0), pa1 = quad_get<kQuadXor1>(a1);
+      const float pd0 = quad_get<kQuadXor1>(d0), pd1 = quad_get<kQuadXor1>(d1);
+      const bool lo = r0 == 0;
+      wy[0] = lo ? a0 : pa0;
+      wy[1] = lo ? a1 : pa1;
+      wy[2] = lo ? pa0 : a0;
+      wy[3] = lo ? pa1 : a1;
+      dwy[0] = lo ? d0 : pd0;
+      dwy[1] = lo ? d1 : pd1;
+      dwy[2] = lo ? pd0 : d0;
+      dwy[3] = lo ? pd1 : d1;
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float ay = (float)(refy - 1 + i) - ry;
+        wy[i] = delta_kernel(ib.kernel, ay);
+        dwy[i] = delta_kernel_deriv(ib.kernel, ay);
+      }
     }
     base = (refx - 1 + r0) * S + (refy - 1);
 #pragma unroll
@@ -466,10 +611,35 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
     uby += ay;
     __syncthreads();
   }
+  // ---- hand-off: the reverse bulk kernel of this step has published the
+  // force-window adjoint and fbar_t on the window and wall columns
+  {
+    // published columns: the window; the wall columns 0, nx - 2, nx - 1 when
+    // this kernel runs the walls of step t - 1
+    int npub = S;
+    if (!adj_walls_clear(xc, S, P.nx)) npub += (xc > 0) + (xc + S <= P.nx - 2) + (xc + S <= P.nx - 1);
+    if (tid == 0) {
+      const int target = __hip_atomic_load(&P.sync[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + P.nby * npub;
+      adj_wait(P.sync, 0, target);
+      __hip_atomic_store(&P.sync[1], target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+  }
   // ---- reverse
+  // the force-window adjoint at the marked cells, staged in LDS over the spread
+  // grid (its marked cells, the only ones read, are zeroed by every spread)
+  {
+    float2 *gst = reinterpret_cast<float2 *>(acc);
+    for (int idx = tid; idx < n_list; idx += nt) {
+      const int k = list[idx];
+      gst[k] = make_float2(ld_agent(&gbar[k]), ld_agent(&gbar[SS + k]));
+    }
+    __syncthreads();
+  }
   // g = W^T F (scatter rule); H = -sum F + a area
   float Fbx, Fby;
-  gather_with([&](int gi) { return make_float2(gbar[gi], gbar[SS + gi]); }, true, Fx, Fy, Fbx, Fby);
+  gather_with([&](int gi) { return reinterpret_cast<const float2 *>(acc)[gi]; }, true, Fx, Fy, Fbx, Fby);
+  __syncthreads();   // the staged values are read before a spread zeroes the grid
   Fbx -= (float)s_misc[0];
   Fby -= (float)s_misc[1];
   float ubbx = 0.f, ubby = 0.f, Vbx = 0.f, Vby = 0.f;
@@ -504,29 +674,62 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
   }
   spread_acc(false, ubbx, ubby);
   // fbar_t += (c_i - u) . ubar / rho on the window cells the stencils reach
-  for (int idx = tid; idx < n_list; idx += nt) {
-    const int k = list[idx];
-    const float2 ub0 = acc_get(acc, SS, k);
-    if (ub0.x == 0.f && ub0.y == 0.f) continue;
-    const size_t cell = (size_t)(xc + k / S) * P.ny + (yc + k % S);
-    float rho = 0.f;
+  // two cells per thread and pass, every load issued before the first store
+  // (agent-scope loads and stores stay in program order: a load after a
+  // store would wait for it)
+  for (int i0 = tid; i0 < n_list; i0 += 2 * nt) {
+    float fv[2][9], fb[2][9];
+    float2 ub[2];
+    size_t cell[2];
+    bool on[2];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) rho += ft[i * n + cell];
-    const float2 u = uw[k];
+    for (int h = 0; h < 2; ++h) {
+      const int idx = i0 + h * nt;
+      const int k = idx < n_list ? list[idx] : 0;
+      ub[h] = idx < n_list ? acc_get(acc, SS, k) : make_float2(0.f, 0.f);
+      on[h] = ub[h].x != 0.f || ub[h].y != 0.f;
+      cell[h] = (size_t)(xc + k / S) * P.ny + (yc + k % S);
+      if (on[h]) {
 #pragma unroll
-    for (int i = 0; i < 9; ++i)
-      fbar[i * n + cell] += (ub0.x * ((float)D2Q9::cx[i] - u.x) + ub0.y * ((float)D2Q9::cy[i] - u.y)) / rho;
+        for (int i = 0; i < 9; ++i) {
+          fv[h][i] = ft[i * n + cell[h]];
+          fb[h][i] = ld_agent(&fbar[i * n + cell[h]]);
+        }
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      if (!on[h]) continue;
+      const int k = list[i0 + h * nt];
+      float rho = 0.f;
+#pragma unroll
+      for (int i = 0; i < 9; ++i) rho += fv[h][i];
+      const float2 u = uw[k];
+#pragma unroll
+      for (int i = 0; i < 9; ++i)
+        st_agent(&fbar[i * n + cell[h]],
+                 fb[h][i] + (ub[h].x * ((float)D2Q9::cx[i] - u.x) + ub[h].y * ((float)D2Q9::cy[i] - u.y)) / rho);
+    }
+  }
+  if (u_in_prev && !adj_walls_clear(xc, S, P.nx)) {
+    // the wall adjoint of step t - 1 when the window touches a wall column
+    // (else k_adj_walls on the bulk stream): after the window update
+    wait_mem();
+    __syncthreads();
+    const double part = adj_walls_rows<true>(fbar, ft, P.nx, P.ny, P.u_in);
+    const double tot = block_sum_d(part, red);
+    if (tid == 0) *u_in_prev = tot;
   }
   // marker-position adjoints: delta (radial direction), displacement; the
   // marker-level values V are held by every thread of a marker, counted once
   const bool lead = act && l.first_row() == 0;
   float rxm = rxb, rym = ryb;
   pair_sum(rxm, rym);
-  if (lead) grad_delta[m] += (double)rxm * dir_x[m] + (double)rym * dir_y[m];
-  const double srx = block_sum_d(act ? (double)rxb : 0.0, red);
-  const double sry = block_sum_d(act ? (double)ryb : 0.0, red);
-  const double svx = block_sum_d(lead ? (double)Vbx : 0.0, red);
-  const double svy = block_sum_d(lead ? (double)Vby : 0.0, red);
+  if (lead) grad_delta[m] = gd0 + ((double)rxm * dxm + (double)rym * dym);
+  double sums[4] = {act ? (double)rxb : 0.0, act ? (double)ryb : 0.0, lead ? (double)Vbx : 0.0,
+                    lead ? (double)Vby : 0.0};
+  block_sum4_d(sums, red4);
+  const double srx = sums[0], sry = sums[1], svx = sums[2], svy = sums[3];
   if (tid == 0) {
     adj_body[0] = s_ab[0];
     adj_body[1] = s_ab[1];
@@ -536,6 +739,11 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
     adj_body[5] = s_ab[5] + sry;
     adj_body[6] += s_misc[2];
   }
+  // every agent-scope store of fbar_t has completed: the reverse bulk kernel of
+  // step t - 1 may read it
+  wait_mem();
+  __syncthreads();
+  if (tid == 0) __hip_atomic_store(&P.sync[2], done_value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __global__ void k_adj_final(const double *hist_d, int T, const double *adj_body, const double *u_in_part, double *out) {
@@ -571,7 +779,8 @@ int validate_adj(const VsxLbm *p, const VsxIB2D *ib, const VsxAdj2D *a) {
   VSX_REQUIRE(ib->n_iter >= 1 && ib->n_iter <= kMaxIter, "the adjoint takes 1..%d MDF iterations", kMaxIter);
   VSX_REQUIRE(a->n_steps >= 1 && a->seg >= 1, "bad step count / segment");
   VSX_REQUIRE(a->dir_x && a->dir_y && a->ckpt && a->ckpt_body && a->tape_f && a->tape_g && a->tape_body && a->adj_a &&
-                  a->adj_b && a->grid && a->adj_body && a->hist_d && a->u_in_part && a->grad_delta && a->out,
+                  a->adj_b && a->grid && a->adj_body && a->hist_d && a->u_in_part && a->grad_delta && a->out &&
+                  a->sync,
               "null adjoint buffer");
   VSX_REQUIRE(p->nx >= 4, "grid too small");
   return VSX_OK;
@@ -579,15 +788,16 @@ int validate_adj(const VsxLbm *p, const VsxIB2D *ib, const VsxAdj2D *a) {
 
 template <int NIT>
 void launch_adj_ib(const VsxIB2D &ib, const AdjIB &P, const float *ft, const float *gbar, float *fbar,
-                   const double *body, const double *body1, const VsxAdj2D &a, int max_lds, hipStream_t s) {
+                   const double *body, const double *body1, const VsxAdj2D &a, double *u_in_prev, int done_value,
+                   int max_lds, hipStream_t s) {
   const size_t keep = sizeof(float) * (size_t)adj_lds_layout(ib.win, ib.n_markers, NIT, true).total;
   const size_t drop = sizeof(float) * (size_t)adj_lds_layout(ib.win, ib.n_markers, NIT, false).total;
   if (keep + 2048 <= (size_t)max_lds)
     hipLaunchKernelGGL((k_adj_ib<NIT, true>), dim3(1), dim3(kIbThreads), keep, s, ib, P, ft, gbar, fbar, body, body1,
-                       a.adj_body, a.grad_delta, a.dir_x, a.dir_y);
+                       a.adj_body, a.grad_delta, a.dir_x, a.dir_y, u_in_prev, done_value);
   else
     hipLaunchKernelGGL((k_adj_ib<NIT, false>), dim3(1), dim3(kIbThreads), drop, s, ib, P, ft, gbar, fbar, body, body1,
-                       a.adj_body, a.grad_delta, a.dir_x, a.dir_y);
+                       a.adj_body, a.grad_delta, a.dir_x, a.dir_y, u_in_prev, done_value);
 }
 
 int viv_adjoint(const VsxLbm *p, const VsxIB2D *ib, const VsxAdj2D *a, const float *f0, void *wa, void *wb,
@@ -639,23 +849,39 @@ int viv_adjoint(const VsxLbm *p, const VsxIB2D *ib, const VsxAdj2D *a, const flo
   for (int k = 0; k < 9; ++k) PB.rates[k] = p->mrt_rates[k];
   PB.win_x0 = ib->win_x0;
   PB.win_y0 = ib->win_y0;
+  const dim3 bgrid((p->ny + 255) / 256, p->nx);
+  VSX_HIP(hipMemsetAsync(a->sync, 0, sizeof(int32_t) * 4, s));
+  PB.sync = a->sync;
   AdjIB PI;
   PI.nx = p->nx;
   PI.ny = p->ny;
   PI.inv_T = 1.0 / T;
-  const dim3 bgrid((p->ny + 255) / 256, p->nx);
-  // The recompute of segment sg - 1 (second stream) overlaps the reverse pass of
-  // segment sg (caller's stream): both are chains of small latency-bound
-  // kernels.  Tapes are double-buffered by segment parity.
-  hipStream_t s2 = nullptr;
-  VSX_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
-  std::vector<hipEvent_t> ev_rec(nseg, nullptr), ev_rev(nseg, nullptr);
-  hipEvent_t ev_fwd = nullptr;
+  PI.sync = a->sync;
+  PI.nby = (int)bgrid.x;
+  PI.u_in = p->walls[0].ux;
+  // Three streams: the recompute of segment sg - 1 (s2) overlaps the reverse
+  // pass of segment sg; in the reverse pass the bulk and wall kernels run on
+  // the caller's stream and the IB kernels on s3, each IB kernel replaying the
+  // forward MDF while the bulk kernel of its step runs and taking that
+  // kernel's output through the hand-off words a->sync (k_adj_bulk,
+  // k_adj_ib).  Every kernel that waits on another is enqueued after it, so
+  // the waits end whether or not the streams share a hardware queue.  s3 and
+  // s2 are the library's two streams of the device (the ones the pipelined
+  // forward pass runs its IB kernels on, and the 3D bulk stream), created
+  // once: a stream created per call may land on the caller's hardware queue,
+  // where the two kernels of a step would run one after the other.  Tapes
+  // are double-buffered by segment parity.
+  PipeRes *res = nullptr;
+  if ((rc = pipe_res(&res))) return rc;
+  hipStream_t s2 = res->b, s3 = res->s;
+  std::vector<hipEvent_t> ev_rec(nseg, nullptr), ev_rev(nseg, nullptr), ev_rev3(nseg, nullptr);
+  hipEvent_t ev_fwd = nullptr, ev_end = nullptr;
   auto cleanup = [&]() {
     for (auto e : ev_rec) if (e) (void)hipEventDestroy(e);
     for (auto e : ev_rev) if (e) (void)hipEventDestroy(e);
+    for (auto e : ev_rev3) if (e) (void)hipEventDestroy(e);
     if (ev_fwd) (void)hipEventDestroy(ev_fwd);
-    (void)hipStreamDestroy(s2);
+    if (ev_end) (void)hipEventDestroy(ev_end);
   };
   auto check = [&](hipError_t e) { return e == hipSuccess ? VSX_OK : fail(VSX_ERR_HIP, "adjoint: %s", hipGetErrorString(e)); };
   auto tf = [&](int sg, int i) { return a->tape_f + ((size_t)(sg & 1) * (K + 1) + i) * fsz; };
@@ -680,38 +906,56 @@ int viv_adjoint(const VsxLbm *p, const VsxIB2D *ib, const VsxAdj2D *a, const flo
     if (!r) r = check(hipEventRecord(ev_rec[sg], s2));
     return r;
   };
-  rc = check(hipEventCreateWithFlags(&ev_fwd, hipEventDisableTiming));
-  if (!rc) rc = check(hipEventRecord(ev_fwd, s));
+  auto event = [&](hipEvent_t &e, hipStream_t st) {
+    int r = check(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    return r ? r : check(hipEventRecord(e, st));
+  };
+  rc = event(ev_fwd, s);
   if (!rc) rc = check(hipStreamWaitEvent(s2, ev_fwd, 0));
+  if (!rc) rc = check(hipStreamWaitEvent(s3, ev_fwd, 0));
   if (!rc) rc = recompute(nseg - 1);
+  if (!rc) {
+    // the walls of the last step (those of every other step run inside the IB
+    // kernel of the step after it)
+    const int tl = (nseg - 1) * K, nl = T - tl;
+    rc = check(hipStreamWaitEvent(s, ev_rec[nseg - 1], 0));
+    hipLaunchKernelGGL(k_adj_walls, dim3(1), dim3(1024), 0, s, cur, (const float *)tf(nseg - 1, nl), p->nx, p->ny,
+                       p->walls[0].ux, a->u_in_part + (T - 1), (const double *)nullptr, ib->win_x0, ib->win);
+  }
   for (int sg = nseg - 1; sg >= 0 && !rc; --sg) {
     const int t0 = sg * K, nn = std::min(K, T - t0);
     rc = check(hipStreamWaitEvent(s, ev_rec[sg], 0));
+    if (!rc) rc = check(hipStreamWaitEvent(s3, ev_rec[sg], 0));
     for (int i = nn - 1; i >= 0 && !rc; --i) {
       const int t = t0 + i;
-      const float *ft = tf(sg, i), *ft1 = tf(sg, i + 1);
+      const float *ft = tf(sg, i);
       const double *bt = tb(sg, i), *bt1 = tb(sg, i + 1);
-      hipLaunchKernelGGL(k_adj_walls, dim3(1), dim3(1024), 0, s, cur, ft1, p->nx, p->ny, p->walls[0].ux,
-                         a->u_in_part + t);
       hipLaunchKernelGGL(k_adj_bulk, bgrid, dim3(256), 0, s, PB, (const float *)cur, nxt, ft,
-                         (const float *)tg(sg, i), bt, a->grid);
+                         (const float *)tg(sg, i), bt, bt1, a->grid, T - 1 - t);
+      double *u_prev = t > 0 ? a->u_in_part + (t - 1) : nullptr;
+      if (u_prev)
+        hipLaunchKernelGGL(k_adj_walls, dim3(1), dim3(1024), 0, s, nxt, ft, p->nx, p->ny, p->walls[0].ux, u_prev, bt,
+                           ib->win_x0, ib->win);
       switch (ib->n_iter) {
-        case 1: launch_adj_ib<1>(*ib, PI, ft, a->grid, nxt, bt, bt1, *a, max_lds, s); break;
-        case 2: launch_adj_ib<2>(*ib, PI, ft, a->grid, nxt, bt, bt1, *a, max_lds, s); break;
-        case 3: launch_adj_ib<3>(*ib, PI, ft, a->grid, nxt, bt, bt1, *a, max_lds, s); break;
-        default: launch_adj_ib<4>(*ib, PI, ft, a->grid, nxt, bt, bt1, *a, max_lds, s); break;
+        case 1: launch_adj_ib<1>(*ib, PI, ft, a->grid, nxt, bt, bt1, *a, u_prev, T - t, max_lds, s3); break;
+        case 2: launch_adj_ib<2>(*ib, PI, ft, a->grid, nxt, bt, bt1, *a, u_prev, T - t, max_lds, s3); break;
+        case 3: launch_adj_ib<3>(*ib, PI, ft, a->grid, nxt, bt, bt1, *a, u_prev, T - t, max_lds, s3); break;
+        default: launch_adj_ib<4>(*ib, PI, ft, a->grid, nxt, bt, bt1, *a, u_prev, T - t, max_lds, s3); break;
       }
       rc = check(hipGetLastError());
       std::swap(cur, nxt);
     }
-    if (!rc) rc = check(hipEventCreateWithFlags(&ev_rev[sg], hipEventDisableTiming));
-    if (!rc) rc = check(hipEventRecord(ev_rev[sg], s));
+    if (!rc) rc = event(ev_rev[sg], s);
+    if (!rc) rc = event(ev_rev3[sg], s3);
     if (!rc && sg > 0) {
       // tape parity (sg - 1) & 1 was last read by the reverse pass of segment sg + 1
       if (sg + 1 < nseg) rc = check(hipStreamWaitEvent(s2, ev_rev[sg + 1], 0));
+      if (!rc && sg + 1 < nseg) rc = check(hipStreamWaitEvent(s2, ev_rev3[sg + 1], 0));
       if (!rc) rc = recompute(sg - 1);
     }
   }
+  if (!rc) rc = event(ev_end, s3);
+  if (!rc) rc = check(hipStreamWaitEvent(s, ev_end, 0));
   cleanup();
   if (rc) return rc;
   hipLaunchKernelGGL(k_adj_final, dim3(1), dim3(1024), 0, s, (const double *)a->hist_d, T, (const double *)a->adj_body,

@@ -690,7 +690,8 @@ class VIVAdjoint:
         self.buf = dict(ckpt=z32(nseg, 9 * n), ckpt_body=z64(nseg, 6), tape_f=z32(2, self.seg + 1, 9 * n),
                         tape_g=z32(2, self.seg, 2 * S * S), tape_body=z64(2, self.seg + 1, 6), adj_a=z32(9 * n),
                         adj_b=z32(9 * n), grid=z32(2 * S * S), adj_body=z64(8), hist_d=z64(self.T, 2),
-                        u_in_part=z64(self.T), grad_delta=z64(M), out=z64(3))
+                        u_in_part=z64(self.T), grad_delta=z64(M), out=z64(3),
+                        sync=torch.zeros(4, dtype=torch.int32, device=dev))
         self.f_tmp = z32(9 * n)
         a = L.VsxAdj2D()
         a.n_steps, a.seg = self.T, self.seg
@@ -720,6 +721,8 @@ class VIVAdjoint:
         L.call("vsx_viv2d_adjoint", ctypes.byref(lb.p), ctypes.byref(body.ib), ctypes.byref(self.adj),
                L.ptr(self.f0), L.ptr(wa), L.ptr(wb), L.ptr(self.f_tmp))
         out = self.buf["out"].cpu().numpy()
+        if int(self.buf["sync"][3].item()) != 0:
+            raise RuntimeError("vsx_viv2d_adjoint: a wait between the reverse kernels timed out; the gradient is invalid")
         return float(out[0]), self.buf["grad_delta"].clone(), float(out[1]), float(out[2])
 
     def objective(self, delta, stiffness, u_in):
