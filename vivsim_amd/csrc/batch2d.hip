@@ -105,7 +105,9 @@ k_viv_batch(const BatchCase *__restrict__ tab, const uint32_t *__restrict__ bmap
   store_cell<D2Q9, DST>(c, d_case, cell, f);
 }
 
-// Split mode: the IB workgroup of every case (one workgroup per case).
+// Split mode: the IB workgroup of every case (one workgroup per case), the
+// single-case IB workgroup (marked-cell window pull, fixed-point LDS spreads,
+// several threads per marker); dynamic LDS >= 16 S^2 bytes for every case.
 template <class SRC, bool PULL>
 __global__ __launch_bounds__(kIbThreads) void k_viv_batch_ib(const BatchCase *__restrict__ tab,
                                                              const typename SRC::T *__restrict__ src, int step,
@@ -113,8 +115,9 @@ __global__ __launch_bounds__(kIbThreads) void k_viv_batch_ib(const BatchCase *__
   extern __shared__ float lds[];
   const BatchCase &bc = tab[blockIdx.x];
   const int gstep = *clock + step, slot = gstep & 1;
-  ib_block<SRC, PULL, 1, 4, 3, false, 2, true>(bc.c, bc.ib, src + bc.f_off, gstep < bc.hist_cap ? gstep : -1, gstep, 0,
-                                               bc.ib.body + 6 * slot, bc.ib.body + 6 * (slot ^ 1), lds);
+  ib_atomic<SRC, PULL ? kWinPull : kWinLocal, 2, 3, false>(bc.c, bc.ib, src + bc.f_off, nullptr,
+                                                          gstep < bc.hist_cap ? gstep : -1, gstep, nullptr,
+                                                          bc.ib.body + 6 * slot, bc.ib.body + 6 * (slot ^ 1), lds);
 }
 
 // Dynamic LDS above which the IB workgroups run in their own launch (two
@@ -201,7 +204,7 @@ int batch_prepare(const VsxVivCase *cases, int n, void *table, int32_t *clock, V
   std::vector<BatchCase> tab(n);
   std::vector<uint32_t> outer, band;
   int64_t pops = 0;
-  int lds = 0;
+  int lds = 0, lds_split = 0;
   for (int k = 0; k < n; ++k) {
     const VsxVivCase &vc = cases[k];
     int rc = validate_lbm<D2Q9>(&vc.lbm);
@@ -236,7 +239,10 @@ int batch_prepare(const VsxVivCase *cases, int n, void *table, int32_t *clock, V
       }
     pops = std::max(pops, vc.f_offset + (int64_t)9 * vc.lbm.nx * vc.lbm.ny);
     lds = std::max(lds, 4 * ib_lds_words(vc.ib.win, vc.ib.n_markers, true));
+    lds_split = std::max(lds_split, 4 * ib_atomic_lds_words(vc.ib.win));
   }
+  // split mode (see k_viv_batch): the IB launch takes the LDS of k_viv_batch_ib
+  if (lds > kBatchSplitLds) lds = std::max(lds, lds_split);
   VSX_REQUIRE(lds + 1024 <= max_lds, "IB windows need %d B of LDS, device allows %d", lds, max_lds);
   std::vector<char> host(sizeof(BatchCase) * n + 4 * (outer.size() + band.size()));
   std::memcpy(host.data(), tab.data(), sizeof(BatchCase) * n);
