@@ -350,7 +350,7 @@ __host__ __device__ inline AdjLds adj_lds_layout(int S, int M, int nit, bool kee
 // the order of the additions), with the index rule of the operation for
 // stencils that leave the window.  KEEP: the forward replay's spread grids
 // A_k stay in LDS for the reverse loop, else they are spread again.
-template <int NIT, bool KEEP>
+template <int NIT, bool KEEP, int TPM>
 __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const float *__restrict__ ft,
                                                  const float *__restrict__ gbar, float *__restrict__ fbar,
                                                  const double *__restrict__ body, const double *__restrict__ body1,
@@ -370,7 +370,7 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
   __shared__ double s_body[6], s_ab[6], s_misc[4];
   __shared__ double red[16], red4[4 * 16];
   __shared__ int s_org[4], s_scan[kIbThreads / 64 + 1];
-  const IbLane l = ib_lane(M);
+  const IbLane l = ib_lane<TPM>(M);
   const bool act = l.mk < M;
   const int m = l.mk;
 
@@ -792,12 +792,21 @@ void launch_adj_ib(const VsxIB2D &ib, const AdjIB &P, const float *ft, const flo
                    int max_lds, hipStream_t s) {
   const size_t keep = sizeof(float) * (size_t)adj_lds_layout(ib.win, ib.n_markers, NIT, true).total;
   const size_t drop = sizeof(float) * (size_t)adj_lds_layout(ib.win, ib.n_markers, NIT, false).total;
-  if (keep + 2048 <= (size_t)max_lds)
-    hipLaunchKernelGGL((k_adj_ib<NIT, true>), dim3(1), dim3(kIbThreads), keep, s, ib, P, ft, gbar, fbar, body, body1,
-                       a.adj_body, a.grad_delta, a.dir_x, a.dir_y, u_in_prev, done_value);
-  else
-    hipLaunchKernelGGL((k_adj_ib<NIT, false>), dim3(1), dim3(kIbThreads), drop, s, ib, P, ft, gbar, fbar, body, body1,
-                       a.adj_body, a.grad_delta, a.dir_x, a.dir_y, u_in_prev, done_value);
+  // one kernel per lane mapping (threads per marker): a kernel holds the code of one only
+  auto go = [&](auto tpm) {
+    constexpr int T = decltype(tpm)::value;
+    if (keep + 2048 <= (size_t)max_lds)
+      hipLaunchKernelGGL((k_adj_ib<NIT, true, T>), dim3(1), dim3(kIbThreads), keep, s, ib, P, ft, gbar, fbar, body,
+                         body1, a.adj_body, a.grad_delta, a.dir_x, a.dir_y, u_in_prev, done_value);
+    else
+      hipLaunchKernelGGL((k_adj_ib<NIT, false, T>), dim3(1), dim3(kIbThreads), drop, s, ib, P, ft, gbar, fbar, body,
+                         body1, a.adj_body, a.grad_delta, a.dir_x, a.dir_y, u_in_prev, done_value);
+  };
+  switch (ib_tpm(ib.n_markers, kIbThreads)) {
+    case 4: go(std::integral_constant<int, 4>()); break;
+    case 2: go(std::integral_constant<int, 2>()); break;
+    default: go(std::integral_constant<int, 1>()); break;
+  }
 }
 
 int viv_adjoint(const VsxLbm *p, const VsxIB2D *ib, const VsxAdj2D *a, const float *f0, void *wa, void *wb,

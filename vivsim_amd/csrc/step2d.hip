@@ -6,15 +6,16 @@ namespace vsx {
 
 // Stand-alone IB kernel (the first step of a run, and the non-fused path):
 // the fixed-point LDS spread of the fused steps for up to one marker per
-// thread, the binned spread for more markers (MPT > 1).
-template <class SRC, bool PULL, int MPT>
+// thread, the binned spread for more markers (MPT > 1); TPM: threads per
+// marker (ib_tpm) of the former.
+template <class SRC, bool PULL, int MPT, int TPM = 1>
 __global__ __launch_bounds__(kIbThreads) void k_viv_ib(StepCtx c_arg, VsxIB2D ib,
                                                         const typename SRC::T *__restrict__ src, int step) {
   const StepCtx &c = kernel_ctx();
   extern __shared__ float lds[];
   const int gstep = ib.step0 + step, slot = gstep & 1;
   if constexpr (MPT == 1)
-    ib_atomic<SRC, PULL ? kWinPull : kWinLocal, 2, 3, true>(c, ib, src, nullptr, step, gstep, nullptr,
+    ib_atomic<SRC, PULL ? kWinPull : kWinLocal, 2, 3, true, TPM>(c, ib, src, nullptr, step, gstep, nullptr,
                                                             ib.body + 6 * slot, ib.body + 6 * (slot ^ 1), lds);
   else
     ib_block<SRC, PULL, MPT, 4, 0, false, 4>(c, ib, src, step, gstep, 0, ib.body + 6 * slot,
@@ -31,7 +32,11 @@ int launch_viv_ib(const StepCtx &c, const VsxIB2D &ib, const void *src, int step
   if (int rc = max_lds_per_block(&max_lds)) return rc;
   const size_t lds_atomic = 4 * (size_t)ib_split_lds_words(ib.win);
   if (ib.n_markers <= kIbThreads && lds_atomic + 2048 <= (size_t)max_lds) {
-    hipLaunchKernelGGL((k_viv_ib<SRC, PULL, 1>), dim3(1), dim3(kIbThreads), lds_atomic, s, c, ib, S, step);
+    switch (ib_tpm(ib.n_markers, kIbThreads)) {
+      case 4: hipLaunchKernelGGL((k_viv_ib<SRC, PULL, 1, 4>), dim3(1), dim3(kIbThreads), lds_atomic, s, c, ib, S, step); break;
+      case 2: hipLaunchKernelGGL((k_viv_ib<SRC, PULL, 1, 2>), dim3(1), dim3(kIbThreads), lds_atomic, s, c, ib, S, step); break;
+      default: hipLaunchKernelGGL((k_viv_ib<SRC, PULL, 1, 1>), dim3(1), dim3(kIbThreads), lds_atomic, s, c, ib, S, step); break;
+    }
   } else {
     const size_t lds = 4 * (size_t)ib_lds_words(ib.win, ib.n_markers, false);
     hipLaunchKernelGGL((k_viv_ib<SRC, PULL, kMarkersPerThread>), dim3(1), dim3(kIbThreads), lds, s, c, ib, S, step);

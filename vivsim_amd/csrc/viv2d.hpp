@@ -953,11 +953,16 @@ __device__ __forceinline__ float marker_sum(float v, const IbLane &l) {
   if (l.tpm == 4) v += quad_get<kQuadXor2>(v);
   return v;
 }
+// Threads per marker for M markers and nt threads (host and device).
+__host__ __device__ inline int ib_tpm(int M, int nt) { return 4 * M <= nt ? 4 : (2 * M <= nt ? 2 : 1); }
+// TPM: the mapping fixed at compile time (ib_tpm(M, blockDim.x), chosen by the
+// launcher), so that a kernel holds the code of one mapping only; 0: chosen here.
+template <int TPM = 0>
 __device__ __forceinline__ IbLane ib_lane(int M) {
   const int tid = threadIdx.x, nt = blockDim.x, nwave = nt >> 6, lane = tid & 63, wave = tid >> 6;
   IbLane l;
   // uniform: kept in a scalar register
-  l.tpm = __builtin_amdgcn_readfirstlane(4 * M <= nt ? 4 : 2 * M <= nt ? 2 : 1);
+  l.tpm = TPM ? TPM : __builtin_amdgcn_readfirstlane(ib_tpm(M, nt));
   if (l.tpm == 4) {
     l.mk = (lane >> 2) * nwave + wave;
     l.r0 = lane & 3;
@@ -1158,7 +1163,7 @@ constexpr int kWinLocal = 1;   // read src in place (a post-streaming state)
 constexpr int kWinHalo = 2;    // pull from the FP32 hand-off buffer `halo` [9][S + 4][S + 4] whose origin
                                // is the previous step's window origin - 2 (ib.win_origin[0..1])
 
-template <class SRC, int WSRC, int UW, int TAG, bool SPLIT = false>
+template <class SRC, int WSRC, int UW, int TAG, bool SPLIT = false, int TPM = 0>
 __device__ __forceinline__ void ib_atomic(const StepCtx &c, const VsxIB2D &ib, const typename SRC::T *__restrict__ src,
                                           const float *halo, int hidx, int gstep, int32_t *go_flag,
                                           const double *body_in, double *body_out, float *lds,
@@ -1169,7 +1174,7 @@ __device__ __forceinline__ void ib_atomic(const StepCtx &c, const VsxIB2D &ib, c
   __shared__ double s_red[2][kIbThreads / 64];
   __shared__ int s_scan[kIbThreads / 64 + 1];
   const int tid = threadIdx.x, nt = blockDim.x;
-  const IbLane l = ib_lane(M);
+  const IbLane l = ib_lane<TPM>(M);
   const bool live = l.mk < M;
 
   // the first IB kernel of a pipelined run (START) waits for the run's first

@@ -283,7 +283,7 @@ k_viv_fused(StepCtx c_arg, VsxIB2D ib, const typename SRC::T *__restrict__ src, 
 // caller's post-streaming FP32 field in place (kWinLocal).  One block of 1024
 // threads; it runs on a compute unit of its own (the IB stream's CU mask), so
 // its LDS and registers do not constrain the bulk kernel's occupancy.
-template <int WSRC>
+template <int WSRC, int TPM>
 __global__ __launch_bounds__(kIbThreads) void k_ib_pipe(StepCtx c_arg, VsxIB2D ib, const float *__restrict__ src,
                                                        int step) {
   const StepCtx &c = kernel_ctx();
@@ -295,7 +295,7 @@ __global__ __launch_bounds__(kIbThreads) void k_ib_pipe(StepCtx c_arg, VsxIB2D i
   // the pull waits for that pass's hand-off flag
   // (the first IB kernel of a run waits for the run's first bulk pass to start:
   // ib.win_origin[7] = step0 + 1)
-  ib_atomic<Store<VSX_STORE_F32>, WSRC, 2, 2, true>(c, ib, src, halo, step, gstep, nullptr, ib.body + 6 * slot,
+  ib_atomic<Store<VSX_STORE_F32>, WSRC, 2, 2, true, TPM>(c, ib, src, halo, step, gstep, nullptr, ib.body + 6 * slot,
                                                      ib.body + 6 * (slot ^ 1), lds,
                                                      WSRC == kWinHalo ? &ib.win_origin[6] : &ib.win_origin[7],
                                                      WSRC == kWinHalo ? gstep : gstep + 1);
@@ -451,8 +451,16 @@ inline int launch_ib_pipe(const StepCtx &c, const VsxIB2D &ib, const float *src,
                           hipStream_t s) {
   const size_t lds = 4 * (size_t)ib_split_lds_words(ib.win);
   const int nt = kIbThreads;
-  if (first) hipLaunchKernelGGL((k_ib_pipe<kWinLocal>), dim3(1), dim3(nt), lds, s, c, ib, src, step);
-  else hipLaunchKernelGGL((k_ib_pipe<kWinHalo>), dim3(1), dim3(nt), lds, s, c, ib, src, step);
+  auto go = [&](auto tpm) {
+    constexpr int T = decltype(tpm)::value;
+    if (first) hipLaunchKernelGGL((k_ib_pipe<kWinLocal, T>), dim3(1), dim3(nt), lds, s, c, ib, src, step);
+    else hipLaunchKernelGGL((k_ib_pipe<kWinHalo, T>), dim3(1), dim3(nt), lds, s, c, ib, src, step);
+  };
+  switch (ib_tpm(ib.n_markers, nt)) {
+    case 4: go(std::integral_constant<int, 4>()); break;
+    case 2: go(std::integral_constant<int, 2>()); break;
+    default: go(std::integral_constant<int, 1>()); break;
+  }
   VSX_CHECK_LAUNCH();
   return VSX_OK;
 }
