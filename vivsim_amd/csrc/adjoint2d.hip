@@ -374,26 +374,32 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
   const bool act = l.mk < M;
   const int m = l.mk;
 
-  // round trip 1: body state, its adjoint, marker inputs
-  if (tid < 6) {
-    s_body[tid] = body[tid];
-    s_ab[tid] = adj_body[tid];
-  }
-  double mxd = 0.0, myd = 0.0, gd0 = 0.0, dxm = 0.0, dym = 0.0;
+  // round trip 1: body state, marker inputs (what the replay needs; the body
+  // adjoint comes from the IB kernel of step t + 1, read after its hand-off)
+  if (tid < 6) s_body[tid] = body[tid];
+  double mxd = 0.0, myd = 0.0, dxm = 0.0, dym = 0.0;
   float ds2 = 0.f;
   if (act) {
     mxd = ib.marker_x[m];
     myd = ib.marker_y[m];
     ds2 = (ib.ds_arr ? ib.ds_arr[m] : ib.ds) * 2.0f;
-    // the marker's gradient so far and its direction, for the update at the end
-    gd0 = grad_delta[m];
     dxm = dir_x[m];
     dym = dir_y[m];
   }
   for (int k = tid; k < (SS + 3) / 4; k += nt) reinterpret_cast<uint32_t *>(mark)[k] = 0u;
   __syncthreads();
   if (tid == 0) {
-    // objective term and Newmark adjoint (dyn.py:5-51), per DOF
+    const int x0 = (int)trunc(ib.win_x0 + s_body[4]), y0 = (int)trunc(ib.win_y0 + s_body[5]);
+    s_org[0] = x0;
+    s_org[1] = y0;
+    s_org[2] = min(max(x0, 0), P.nx - S);
+    s_org[3] = min(max(y0, 0), P.ny - S);
+  }
+  __syncthreads();
+  const int x0 = s_org[0], y0 = s_org[1], xc = s_org[2], yc = s_org[3];
+  // objective term and Newmark adjoint (dyn.py:5-51), per DOF, once the body
+  // adjoint of step t + 1 is in s_ab (thread 0)
+  auto newmark_adjoint = [&]() {
     s_ab[5] += 2.0 * body1[5] * P.inv_T;
     const double dt = ib.dt, c1 = ib.gamma * dt, c2 = ib.beta * dt * dt;
     const double Mk = ib.mass + c1 * ib.damping + c2 * ib.stiffness;
@@ -414,14 +420,7 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
       s_misc[k] = Hb;
     }
     s_misc[2] = kbar;
-    const int x0 = (int)trunc(ib.win_x0 + s_body[4]), y0 = (int)trunc(ib.win_y0 + s_body[5]);
-    s_org[0] = x0;
-    s_org[1] = y0;
-    s_org[2] = min(max(x0, 0), P.nx - S);
-    s_org[3] = min(max(y0, 0), P.ny - S);
-  }
-  __syncthreads();
-  const int x0 = s_org[0], y0 = s_org[1], xc = s_org[2], yc = s_org[3];
+  };
 
   // stencil of this thread's rows (weights and their derivatives); mark the cells it reads
   float wr[4] = {0, 0, 0, 0}, dwr[4] = {0, 0, 0, 0}, wy[4] = {0, 0, 0, 0}, dwy[4] = {0, 0, 0, 0};
@@ -611,18 +610,27 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
     uby += ay;
     __syncthreads();
   }
-  // ---- hand-off: the reverse bulk kernel of this step has published the
+  // ---- hand-offs: the IB kernel of step t + 1 has finished (sync[2]; it ran
+  // on the other IB stream and wrote the body adjoint, the marker gradients
+  // and sync[1]), then the reverse bulk kernel of this step has published the
   // force-window adjoint and fbar_t on the window and wall columns
+  double gd0 = 0.0;
   {
     // published columns: the window; the wall columns 0, nx - 2, nx - 1 when
     // this kernel runs the walls of step t - 1
     int npub = S;
     if (!adj_walls_clear(xc, S, P.nx)) npub += (xc > 0) + (xc + S <= P.nx - 2) + (xc + S <= P.nx - 1);
     if (tid == 0) {
+      adj_wait(P.sync, 2, done_value - 1);
       const int target = __hip_atomic_load(&P.sync[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + P.nby * npub;
       adj_wait(P.sync, 0, target);
       __hip_atomic_store(&P.sync[1], target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+    __syncthreads();
+    if (tid < 6) s_ab[tid] = ld_agent(&adj_body[tid]);
+    if (act && l.first_row() == 0) gd0 = ld_agent(&grad_delta[m]);
+    __syncthreads();
+    if (tid == 0) newmark_adjoint();
     __syncthreads();
   }
   // ---- reverse
@@ -725,19 +733,20 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
   const bool lead = act && l.first_row() == 0;
   float rxm = rxb, rym = ryb;
   pair_sum(rxm, rym);
-  if (lead) grad_delta[m] = gd0 + ((double)rxm * dxm + (double)rym * dym);
+  if (lead) st_agent(&grad_delta[m], gd0 + ((double)rxm * dxm + (double)rym * dym));
   double sums[4] = {act ? (double)rxb : 0.0, act ? (double)ryb : 0.0, lead ? (double)Vbx : 0.0,
                     lead ? (double)Vby : 0.0};
   block_sum4_d(sums, red4);
   const double srx = sums[0], sry = sums[1], svx = sums[2], svy = sums[3];
   if (tid == 0) {
-    adj_body[0] = s_ab[0];
-    adj_body[1] = s_ab[1];
-    adj_body[2] = s_ab[2] + svx;
-    adj_body[3] = s_ab[3] + svy;
-    adj_body[4] = s_ab[4] + srx;
-    adj_body[5] = s_ab[5] + sry;
-    adj_body[6] += s_misc[2];
+    // agent-scope stores: read by the IB kernel of step t - 1 on the other IB stream
+    st_agent(&adj_body[0], s_ab[0]);
+    st_agent(&adj_body[1], s_ab[1]);
+    st_agent(&adj_body[2], s_ab[2] + svx);
+    st_agent(&adj_body[3], s_ab[3] + svy);
+    st_agent(&adj_body[4], s_ab[4] + srx);
+    st_agent(&adj_body[5], s_ab[5] + sry);
+    st_agent(&adj_body[6], ld_agent(&adj_body[6]) + s_misc[2]);
   }
   // every agent-scope store of fbar_t has completed: the reverse bulk kernel of
   // step t - 1 may read it
@@ -873,24 +882,30 @@ int viv_adjoint(const VsxLbm *p, const VsxIB2D *ib, const VsxAdj2D *a, const flo
   // the caller's stream and the IB kernels on s3, each IB kernel replaying the
   // forward MDF while the bulk kernel of its step runs and taking that
   // kernel's output through the hand-off words a->sync (k_adj_bulk,
-  // k_adj_ib).  Every kernel that waits on another is enqueued after it, so
-  // the waits end whether or not the streams share a hardware queue.  s3 and
-  // s2 are the library's two streams of the device (the ones the pipelined
-  // forward pass runs its IB kernels on, and the 3D bulk stream), created
-  // once: a stream created per call may land on the caller's hardware queue,
-  // where the two kernels of a step would run one after the other.  Tapes
-  // are double-buffered by segment parity.
+  // k_adj_ib).  The IB kernels alternate between two streams, s3 and s4:
+  // the one of step t replays its forward MDF while the one of step t + 1
+  // runs its reverse half, and waits for it (sync[2]) before it reads the
+  // body adjoint.  Every kernel that waits on another is enqueued after it,
+  // so the waits end whether or not the streams share a hardware queue.
+  // s2, s3 and s4 are the library's streams of the device (the 3D bulk
+  // stream, the one the pipelined forward pass runs its IB kernels on, and a
+  // third), created once: a stream created per call may land on the caller's
+  // hardware queue, where the two kernels of a step would run one after the
+  // other.  Tapes are double-buffered by segment parity.
   PipeRes *res = nullptr;
   if ((rc = pipe_res(&res))) return rc;
-  hipStream_t s2 = res->b, s3 = res->s;
-  std::vector<hipEvent_t> ev_rec(nseg, nullptr), ev_rev(nseg, nullptr), ev_rev3(nseg, nullptr);
-  hipEvent_t ev_fwd = nullptr, ev_end = nullptr;
+  hipStream_t s2 = res->b, s3 = res->s, s4 = res->x;
+  std::vector<hipEvent_t> ev_rec(nseg, nullptr), ev_rev(nseg, nullptr), ev_rev3(nseg, nullptr),
+      ev_rev4(nseg, nullptr);
+  hipEvent_t ev_fwd = nullptr, ev_end3 = nullptr, ev_end4 = nullptr;
   auto cleanup = [&]() {
     for (auto e : ev_rec) if (e) (void)hipEventDestroy(e);
     for (auto e : ev_rev) if (e) (void)hipEventDestroy(e);
     for (auto e : ev_rev3) if (e) (void)hipEventDestroy(e);
+    for (auto e : ev_rev4) if (e) (void)hipEventDestroy(e);
     if (ev_fwd) (void)hipEventDestroy(ev_fwd);
-    if (ev_end) (void)hipEventDestroy(ev_end);
+    if (ev_end3) (void)hipEventDestroy(ev_end3);
+    if (ev_end4) (void)hipEventDestroy(ev_end4);
   };
   auto check = [&](hipError_t e) { return e == hipSuccess ? VSX_OK : fail(VSX_ERR_HIP, "adjoint: %s", hipGetErrorString(e)); };
   auto tf = [&](int sg, int i) { return a->tape_f + ((size_t)(sg & 1) * (K + 1) + i) * fsz; };
@@ -922,6 +937,7 @@ int viv_adjoint(const VsxLbm *p, const VsxIB2D *ib, const VsxAdj2D *a, const flo
   rc = event(ev_fwd, s);
   if (!rc) rc = check(hipStreamWaitEvent(s2, ev_fwd, 0));
   if (!rc) rc = check(hipStreamWaitEvent(s3, ev_fwd, 0));
+  if (!rc) rc = check(hipStreamWaitEvent(s4, ev_fwd, 0));
   if (!rc) rc = recompute(nseg - 1);
   if (!rc) {
     // the walls of the last step (those of every other step run inside the IB
@@ -935,6 +951,7 @@ int viv_adjoint(const VsxLbm *p, const VsxIB2D *ib, const VsxAdj2D *a, const flo
     const int t0 = sg * K, nn = std::min(K, T - t0);
     rc = check(hipStreamWaitEvent(s, ev_rec[sg], 0));
     if (!rc) rc = check(hipStreamWaitEvent(s3, ev_rec[sg], 0));
+    if (!rc) rc = check(hipStreamWaitEvent(s4, ev_rec[sg], 0));
     for (int i = nn - 1; i >= 0 && !rc; --i) {
       const int t = t0 + i;
       const float *ft = tf(sg, i);
@@ -945,26 +962,31 @@ int viv_adjoint(const VsxLbm *p, const VsxIB2D *ib, const VsxAdj2D *a, const flo
       if (u_prev)
         hipLaunchKernelGGL(k_adj_walls, dim3(1), dim3(1024), 0, s, nxt, ft, p->nx, p->ny, p->walls[0].ux, u_prev, bt,
                            ib->win_x0, ib->win);
+      hipStream_t si = (t & 1) ? s3 : s4;
       switch (ib->n_iter) {
-        case 1: launch_adj_ib<1>(*ib, PI, ft, a->grid, nxt, bt, bt1, *a, u_prev, T - t, max_lds, s3); break;
-        case 2: launch_adj_ib<2>(*ib, PI, ft, a->grid, nxt, bt, bt1, *a, u_prev, T - t, max_lds, s3); break;
-        case 3: launch_adj_ib<3>(*ib, PI, ft, a->grid, nxt, bt, bt1, *a, u_prev, T - t, max_lds, s3); break;
-        default: launch_adj_ib<4>(*ib, PI, ft, a->grid, nxt, bt, bt1, *a, u_prev, T - t, max_lds, s3); break;
+        case 1: launch_adj_ib<1>(*ib, PI, ft, a->grid, nxt, bt, bt1, *a, u_prev, T - t, max_lds, si); break;
+        case 2: launch_adj_ib<2>(*ib, PI, ft, a->grid, nxt, bt, bt1, *a, u_prev, T - t, max_lds, si); break;
+        case 3: launch_adj_ib<3>(*ib, PI, ft, a->grid, nxt, bt, bt1, *a, u_prev, T - t, max_lds, si); break;
+        default: launch_adj_ib<4>(*ib, PI, ft, a->grid, nxt, bt, bt1, *a, u_prev, T - t, max_lds, si); break;
       }
       rc = check(hipGetLastError());
       std::swap(cur, nxt);
     }
     if (!rc) rc = event(ev_rev[sg], s);
     if (!rc) rc = event(ev_rev3[sg], s3);
+    if (!rc) rc = event(ev_rev4[sg], s4);
     if (!rc && sg > 0) {
       // tape parity (sg - 1) & 1 was last read by the reverse pass of segment sg + 1
       if (sg + 1 < nseg) rc = check(hipStreamWaitEvent(s2, ev_rev[sg + 1], 0));
       if (!rc && sg + 1 < nseg) rc = check(hipStreamWaitEvent(s2, ev_rev3[sg + 1], 0));
+      if (!rc && sg + 1 < nseg) rc = check(hipStreamWaitEvent(s2, ev_rev4[sg + 1], 0));
       if (!rc) rc = recompute(sg - 1);
     }
   }
-  if (!rc) rc = event(ev_end, s3);
-  if (!rc) rc = check(hipStreamWaitEvent(s, ev_end, 0));
+  if (!rc) rc = event(ev_end3, s3);
+  if (!rc) rc = event(ev_end4, s4);
+  if (!rc) rc = check(hipStreamWaitEvent(s, ev_end3, 0));
+  if (!rc) rc = check(hipStreamWaitEvent(s, ev_end4, 0));
   cleanup();
   if (rc) return rc;
   hipLaunchKernelGGL(k_adj_final, dim3(1), dim3(1024), 0, s, (const double *)a->hist_d, T, (const double *)a->adj_body,

@@ -96,6 +96,7 @@ int pipe_res(PipeRes **out) {
   if (!r.s) {
     VSX_HIP(hipStreamCreateWithFlags(&r.s, hipStreamNonBlocking));
     VSX_HIP(hipStreamCreateWithFlags(&r.b, hipStreamNonBlocking));
+    VSX_HIP(hipStreamCreateWithFlags(&r.x, hipStreamNonBlocking));
     // ordering only: every kernel launch already acquires at its start and
     // releases at its end, so the events need no cache fence of their own
     const unsigned ev = hipEventDisableTiming | hipEventDisableSystemFence;

@@ -22,11 +22,13 @@ inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s);
 // Streams of the pipelined steps (default priority, non-blocking): `s` runs the
 // IB kernels; the 3D step also runs its bulk passes on `b`, forked from the
 // caller's stream and joined back with the events (the 2D step keeps its bulk
-// passes on the caller's stream and needs no events).  One set per device,
-// created on first use and kept for the process.  Calls that use them from
-// several host threads at once on one device are not supported.
+// passes on the caller's stream and needs no events); the adjoint's reverse
+// pass runs its IB kernels on `s` and `x` in turn and its recompute on `b`.
+// One set per device, created on first use and kept for the process.  Calls
+// that use them from several host threads at once on one device are not
+// supported.
 struct PipeRes {
-  hipStream_t s = nullptr, b = nullptr;
+  hipStream_t s = nullptr, b = nullptr, x = nullptr;
   hipEvent_t fork = nullptr, join_s = nullptr, join_b = nullptr;
 };
 int pipe_res(PipeRes **out);
@@ -149,6 +151,12 @@ __device__ __forceinline__ void st_agent(float *p, float v) {
 }
 __device__ __forceinline__ float ld_agent(const float *p) {
   return __hip_atomic_load(const_cast<float *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(double *p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double ld_agent(const double *p) {
+  return __hip_atomic_load(const_cast<double *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 // Wait until every vector memory operation of this thread has completed (its
 // agent-scope stores have reached the coherent level).  The memory clobber
