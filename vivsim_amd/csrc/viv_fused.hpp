@@ -128,7 +128,9 @@ __device__ __forceinline__ void bulk_cell(const StepCtx &c, const typename SRC::
 // With PAIR (kModeFull only): block 0 the IB workgroup, blocks 1..n_win the
 // window tiles, then one block per two bulk tiles; the host launches
 // 1 + ceil((tiles + max window tiles) / 2) blocks and the surplus exits.
-template <class SRC, class DST, int COLL, bool PAIR, int MODE>
+// PIPE: the pipelined form (ib.fused == 2) as a template argument, so that its
+// kernels carry no IB workgroup code (which changes their register allocation).
+template <class SRC, class DST, int COLL, bool PAIR, int MODE, bool PIPE>
 __global__ __attribute__((amdgpu_flat_work_group_size(kTileCells, kTileCells), amdgpu_waves_per_eu(8, 8))) void
 k_viv_fused(StepCtx c_arg, VsxIB2D ib, const typename SRC::T *__restrict__ src, typename DST::T *__restrict__ dst,
             int step, Tiles T) {
@@ -137,19 +139,21 @@ k_viv_fused(StepCtx c_arg, VsxIB2D ib, const typename SRC::T *__restrict__ src, 
   extern __shared__ float lds[];
   const int gstep = ib.step0 + step, slot = gstep & 1;
   const double *body_in = ib.body + 6 * slot;
-  const bool pipe = ib.fused == 2;
+  constexpr bool pipe = PIPE;
   if constexpr (MODE == kModeFirst) {
     // pipelined run: the caller's earlier work on this stream is complete; let
     // the run's first IB kernel start
     if (pipe && blockIdx.x == 0 && threadIdx.x == 0)
       __hip_atomic_store(&ib.win_origin[7], gstep + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
-  if (!pipe && blockIdx.x == 0) {
-    ib_atomic<SRC, MODE == kModeFirst ? kWinLocal : kWinPull, 2, 1>(c, ib, src, nullptr, step, gstep,
-                                                                     &ib.win_origin[6], body_in,
-                                                                     ib.body + 6 * (slot ^ 1), lds);
-    publish_ready(ib, gstep + 1);
-    return;
+  if constexpr (!pipe) {
+    if (blockIdx.x == 0) {
+      ib_atomic<SRC, MODE == kModeFirst ? kWinLocal : kWinPull, 2, 1>(c, ib, src, nullptr, step, gstep,
+                                                                       &ib.win_origin[6], body_in,
+                                                                       ib.body + 6 * (slot ^ 1), lds);
+      publish_ready(ib, gstep + 1);
+      return;
+    }
   }
   __shared__ int s_org[2];
   if (threadIdx.x == 0) {
@@ -405,9 +409,12 @@ int launch_fused_p(const StepCtx &c, const VsxIB2D &ib, const void *src, void *d
     if (grid > n_tiles) grid = n_tiles;
   }
   timer_begin(s);
-  hipLaunchKernelGGL((k_viv_fused<SRC, DST, COLL, PAIR, MODE>), dim3((pipe ? 0 : 1) + grid), dim3(kTileCells), lds,
-                     s, c, ib, static_cast<const typename SRC::T *>(src), static_cast<typename DST::T *>(dst), step,
-                     T);
+  if (pipe)
+    hipLaunchKernelGGL((k_viv_fused<SRC, DST, COLL, PAIR, MODE, true>), dim3(grid), dim3(kTileCells), lds, s, c, ib,
+                       static_cast<const typename SRC::T *>(src), static_cast<typename DST::T *>(dst), step, T);
+  else
+    hipLaunchKernelGGL((k_viv_fused<SRC, DST, COLL, PAIR, MODE, false>), dim3(1 + grid), dim3(kTileCells), lds, s, c,
+                       ib, static_cast<const typename SRC::T *>(src), static_cast<typename DST::T *>(dst), step, T);
   timer_end(s);
   VSX_CHECK_LAUNCH();
   return VSX_OK;
