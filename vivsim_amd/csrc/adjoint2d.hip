@@ -915,16 +915,17 @@ int viv_adjoint(const VsxLbm *p, const VsxIB2D *ib, const VsxAdj2D *a, const flo
     const int t0 = sg * K, nn = std::min(K, T - t0);
     int r = check(hipMemcpyAsync(tf(sg, 0), a->ckpt + (size_t)sg * fsz, sizeof(float) * fsz, hipMemcpyDeviceToDevice, s2));
     if (!r) r = check(hipMemcpyAsync(tb(sg, 0), a->ckpt_body + 6 * (size_t)sg, sizeof(double) * 6, hipMemcpyDeviceToDevice, s2));
-    if (!r) r = check(hipMemcpyAsync(ib->body + 6 * (t0 & 1), a->ckpt_body + 6 * (size_t)sg, sizeof(double) * 6,
-                                     hipMemcpyDeviceToDevice, s2));
     for (int i = 0; i < nn && !r; ++i) {
-      w.step0 = t0 + i;
+      // one step of the separate form with the tape as its body ring: step0 = 0
+      // makes the IB read the body state from body[0..5] = tape row i and write
+      // the next one to body[6..11] = tape row i + 1 (no copy per step)
+      w.step0 = 0;
+      w.body = tb(sg, i);
       w.fused = 0;
       w.hist_d = nullptr;
+      w.hist_h = nullptr;
       w.g_win = tg(sg, i);   // the IB workgroup writes this step's force window into the tape
       r = viv_run(p, &w, tf(sg, i), tf(sg, i + 1), wa, wb, 1, s2);
-      if (!r) r = check(hipMemcpyAsync(tb(sg, i + 1), ib->body + 6 * ((t0 + i + 1) & 1), sizeof(double) * 6,
-                                       hipMemcpyDeviceToDevice, s2));
     }
     if (!r) r = check(hipEventCreateWithFlags(&ev_rec[sg], hipEventDisableTiming));
     if (!r) r = check(hipEventRecord(ev_rec[sg], s2));
