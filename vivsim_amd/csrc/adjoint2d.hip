@@ -915,13 +915,17 @@ int viv_adjoint(const VsxLbm *p, const VsxIB2D *ib, const VsxAdj2D *a, const flo
     const int t0 = sg * K, nn = std::min(K, T - t0);
     int r = check(hipMemcpyAsync(tf(sg, 0), a->ckpt + (size_t)sg * fsz, sizeof(float) * fsz, hipMemcpyDeviceToDevice, s2));
     if (!r) r = check(hipMemcpyAsync(tb(sg, 0), a->ckpt_body + 6 * (size_t)sg, sizeof(double) * 6, hipMemcpyDeviceToDevice, s2));
+    // the one-launch form's ready flags count up from zero within a segment
+    if (!r) r = check(hipMemsetAsync(ib->win_origin, 0, sizeof(int32_t) * 8, s2));
     for (int i = 0; i < nn && !r; ++i) {
-      // one step of the separate form with the tape as its body ring: step0 = 0
-      // makes the IB read the body state from body[0..5] = tape row i and write
-      // the next one to body[6..11] = tape row i + 1 (no copy per step)
-      w.step0 = 0;
+      // one step in the one-launch form (the IB workgroup as block 0 of the
+      // bulk launch, then the streaming pass into the tape), with the tape as
+      // its body ring: an even step0 makes the IB read the body state from
+      // body[0..5] = tape row i and write the next one to body[6..11] = tape
+      // row i + 1 (no copy per step); step0 = 2 i keeps the flags increasing
+      w.step0 = 2 * i;
       w.body = tb(sg, i);
-      w.fused = 0;
+      w.fused = 1;
       w.hist_d = nullptr;
       w.hist_h = nullptr;
       w.g_win = tg(sg, i);   // the IB workgroup writes this step's force window into the tape
