@@ -367,7 +367,7 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
   float2 *Ak = reinterpret_cast<float2 *>(lds + lay.A);    // KEEP: [NIT] forward spreads
   uint8_t *mark = reinterpret_cast<uint8_t *>(lds + lay.mark);
   uint16_t *list = reinterpret_cast<uint16_t *>(lds + lay.list);
-  __shared__ double s_body[6], s_ab[6], s_misc[4];
+  __shared__ double s_body[6], s_ab[7], s_misc[4];
   __shared__ double red[16], red4[4 * 16];
   __shared__ int s_org[4], s_scan[kIbThreads / 64 + 1];
   const IbLane l = ib_lane<TPM>(M);
@@ -627,7 +627,7 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
       __hip_atomic_store(&P.sync[1], target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
-    if (tid < 6) s_ab[tid] = ld_agent(&adj_body[tid]);
+    if (tid < 7) s_ab[tid] = ld_agent(&adj_body[tid]);   // [6]: the stiffness accumulator
     if (act && l.first_row() == 0) gd0 = ld_agent(&grad_delta[m]);
     __syncthreads();
     if (tid == 0) newmark_adjoint();
@@ -746,7 +746,7 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
     st_agent(&adj_body[3], s_ab[3] + svy);
     st_agent(&adj_body[4], s_ab[4] + srx);
     st_agent(&adj_body[5], s_ab[5] + sry);
-    st_agent(&adj_body[6], ld_agent(&adj_body[6]) + s_misc[2]);
+    st_agent(&adj_body[6], s_ab[6] + s_misc[2]);
   }
   // every agent-scope store of fbar_t has completed: the reverse bulk kernel of
   // step t - 1 may read it
