@@ -70,9 +70,9 @@ k_viv_batch(const BatchCase *__restrict__ tab, const uint32_t *__restrict__ bmap
   }
   if (!SPLIT && b < n_cases) {
     if constexpr (IB && !SPLIT) {
-      ib_block<SRC, PHASE == VSX_PHASE_FULL, 1, 2, 2, false, 1, false>(c, bc.ib, src + bc.f_off,
-                                                             gstep < bc.hist_cap ? gstep : -1, gstep, 0,
-                                                             bc.ib.body + 6 * slot, bc.ib.body + 6 * (slot ^ 1), lds);
+      ib_atomic<SRC, PHASE == VSX_PHASE_FULL ? kWinPull : kWinLocal, 2, 2, false>(
+          c, bc.ib, src + bc.f_off, nullptr, gstep < bc.hist_cap ? gstep : -1, gstep, nullptr, bc.ib.body + 6 * slot,
+          bc.ib.body + 6 * (slot ^ 1), lds);
       publish_ready(bc.ib, gstep + 1);
     }
     return;
@@ -204,7 +204,7 @@ int batch_prepare(const VsxVivCase *cases, int n, void *table, int32_t *clock, V
   std::vector<BatchCase> tab(n);
   std::vector<uint32_t> outer, band;
   int64_t pops = 0;
-  int lds = 0, lds_split = 0;
+  int lds = 0;
   for (int k = 0; k < n; ++k) {
     const VsxVivCase &vc = cases[k];
     int rc = validate_lbm<D2Q9>(&vc.lbm);
@@ -238,11 +238,8 @@ int batch_prepare(const VsxVivCase *cases, int n, void *table, int32_t *clock, V
         (tx >= bc.band_lo && tx <= bc.band_hi ? band : outer).push_back(entry);
       }
     pops = std::max(pops, vc.f_offset + (int64_t)9 * vc.lbm.nx * vc.lbm.ny);
-    lds = std::max(lds, 4 * ib_lds_words(vc.ib.win, vc.ib.n_markers, true));
-    lds_split = std::max(lds_split, 4 * ib_atomic_lds_words(vc.ib.win));
+    lds = std::max(lds, 4 * ib_atomic_lds_words(vc.ib.win));   // the IB workgroup's (ib_atomic)
   }
-  // split mode (see k_viv_batch): the IB launch takes the LDS of k_viv_batch_ib
-  if (lds > kBatchSplitLds) lds = std::max(lds, lds_split);
   VSX_REQUIRE(lds + 1024 <= max_lds, "IB windows need %d B of LDS, device allows %d", lds, max_lds);
   std::vector<char> host(sizeof(BatchCase) * n + 4 * (outer.size() + band.size()));
   std::memcpy(host.data(), tab.data(), sizeof(BatchCase) * n);
