@@ -367,7 +367,7 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
   float2 *Ak = reinterpret_cast<float2 *>(lds + lay.A);    // KEEP: [NIT] forward spreads
   uint8_t *mark = reinterpret_cast<uint8_t *>(lds + lay.mark);
   uint16_t *list = reinterpret_cast<uint16_t *>(lds + lay.list);
-  __shared__ double s_body[6], s_ab[7], s_misc[4];
+  __shared__ double s_body[6], s_body1[6], s_ab[7], s_misc[4];
   __shared__ double red[16], red4[4 * 16];
   __shared__ int s_org[4], s_scan[kIbThreads / 64 + 1];
   const IbLane l = ib_lane<TPM>(M);
@@ -376,7 +376,10 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
 
   // round trip 1: body state, marker inputs (what the replay needs; the body
   // adjoint comes from the IB kernel of step t + 1, read after its hand-off)
-  if (tid < 6) s_body[tid] = body[tid];
+  if (tid < 6) {
+    s_body[tid] = body[tid];
+    s_body1[tid] = body1[tid];   // the state after the step, for the Newmark adjoint
+  }
   double mxd = 0.0, myd = 0.0, dxm = 0.0, dym = 0.0;
   float ds2 = 0.f;
   if (act) {
@@ -400,7 +403,7 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
   // objective term and Newmark adjoint (dyn.py:5-51), per DOF, once the body
   // adjoint of step t + 1 is in s_ab (thread 0)
   auto newmark_adjoint = [&]() {
-    s_ab[5] += 2.0 * body1[5] * P.inv_T;
+    s_ab[5] += 2.0 * s_body1[5] * P.inv_T;
     const double dt = ib.dt, c1 = ib.gamma * dt, c2 = ib.beta * dt * dt;
     const double Mk = ib.mass + c1 * ib.damping + c2 * ib.stiffness;
     double kbar = 0.0;
@@ -409,7 +412,7 @@ __global__ __launch_bounds__(1024) void k_adj_ib(VsxIB2D ib, AdjIB P, const floa
       const double ab = on ? s_ab[k] : 0.0, vb = on ? s_ab[2 + k] : 0.0, db = on ? s_ab[4 + k] : 0.0;
       const double a = s_body[k], v = s_body[2 + k], d = s_body[4 + k];
       const double v2 = d + dt * v + dt * dt * (0.5 - ib.beta) * a;
-      const double an = body1[k];
+      const double an = s_body1[k];
       const double anb = ab + c1 * vb + c2 * db;
       const double Hb = anb / Mk;
       kbar += anb * (-v2 - an * c2) / Mk;
