@@ -255,6 +255,29 @@ def test_viv_16bit_pair_blocks_match_oracle(sim, storage, tol):
     assert_close(got, f, tol, f"viv {storage} f (pair blocks)")
 
 
+@pytest.mark.parametrize("storage,tol,nx,ny", [("f32", 5e-5, 2760, 2740), ("f16s", 3e-3, 3880, 3872)])
+def test_viv_streamed_stores_match_oracle(sim, storage, tol, nx, ny):
+    """Pipelined full steps whose work buffer exceeds 256 MiB (the last-level
+    cache) write the field with non-temporal stores: FP32 at 2760 x 2740
+    (272 MB), FP16S at 3880 x 3872 (270 MB; tile pairs, a partial last tile
+    row), 3 steps against the oracle."""
+    kw = dict(nx=nx, ny=ny, cx=256.0, cy=ny / 2)
+    oc, pc = S.VIV2D(**kw), sim.VIVCase(**kw)
+    steps = 3
+    f0 = oc.initial_f(np.float64)
+    v0 = np.array([0.0, 1e-3])
+    solver = sim.VIV2D(pc, storage=storage)
+    assert solver.ib.fused == 2
+    solver.set_body(v=v0)
+    say(f"viv streamed stores {storage} {nx}x{ny}")
+    got, hd, hh = solver.run(dev(f0), steps)
+    solver.check()
+    f, d, v, a, dh, hhist = S.viv_run(oc, f0, steps, v=v0)
+    assert_close(got, f, tol, f"viv {storage} f (streamed stores)")
+    if storage == "f32":
+        assert_close(hh, hhist, 1e-4, "viv f32 h (streamed stores)")
+
+
 def test_viv_split_runs(sim):
     """Runs of 1 + 14 + 25 steps continue the body state and the step counter
     exactly as one run of 40 steps (the oracle's 40-step run)."""

@@ -78,7 +78,8 @@ struct Store<VSX_STORE_F32> {
   using T = float;
   __device__ static inline float cvt(T r, float) { return r; }
   __device__ static inline float ld(const T *p, uint32_t i, float) { return p[i]; }
-  __device__ static inline void st(T *p, uint32_t i, float v, float) { p[i] = v; }
+  __device__ static inline T enc(float v, float) { return v; }
+  __device__ static inline void st(T *p, uint32_t i, float v, float w) { p[i] = enc(v, w); }
   __device__ static inline float rt(float v, float) { return v; }   // value as read back after a store
 };
 
@@ -87,7 +88,8 @@ struct Store<VSX_STORE_F16S> {
   using T = _Float16;
   __device__ static inline float cvt(T r, float w) { return (float)r + w; }
   __device__ static inline float ld(const T *p, uint32_t i, float w) { return (float)p[i] + w; }
-  __device__ static inline void st(T *p, uint32_t i, float v, float w) { p[i] = (_Float16)(v - w); }
+  __device__ static inline T enc(float v, float w) { return (_Float16)(v - w); }
+  __device__ static inline void st(T *p, uint32_t i, float v, float w) { p[i] = enc(v, w); }
   __device__ static inline float rt(float v, float w) { return (float)(_Float16)(v - w) + w; }
 };
 
@@ -104,7 +106,8 @@ struct Store<VSX_STORE_BF16S> {
   using T = uint16_t;
   __device__ static inline float cvt(T r, float w) { return bf16_to_f32(r) + w; }
   __device__ static inline float ld(const T *p, uint32_t i, float w) { return bf16_to_f32(p[i]) + w; }
-  __device__ static inline void st(T *p, uint32_t i, float v, float w) { p[i] = f32_to_bf16(v - w); }
+  __device__ static inline T enc(float v, float w) { return f32_to_bf16(v - w); }
+  __device__ static inline void st(T *p, uint32_t i, float v, float w) { p[i] = enc(v, w); }
   __device__ static inline float rt(float v, float w) { return bf16_to_f32(f32_to_bf16(v - w)) + w; }
 };
 
@@ -131,9 +134,21 @@ struct Store<VSX_STORE_E3M12S> {
   using T = uint16_t;
   __device__ static inline float cvt(T r, float w) { return e3m12_to_f32(r) + w; }
   __device__ static inline float ld(const T *p, uint32_t i, float w) { return e3m12_to_f32(p[i]) + w; }
-  __device__ static inline void st(T *p, uint32_t i, float v, float w) { p[i] = f32_to_e3m12(v - w); }
+  __device__ static inline T enc(float v, float w) { return f32_to_e3m12(v - w); }
+  __device__ static inline void st(T *p, uint32_t i, float v, float w) { p[i] = enc(v, w); }
   __device__ static inline float rt(float v, float w) { return e3m12_to_f32(f32_to_e3m12(v - w)) + w; }
 };
+
+// Storage B written with non-temporal stores: for a field larger than the
+// last-level cache (MI355X: 256 MB of Infinity Cache), whose cells a step
+// writes are evicted before the next step reads them.  Below that size the
+// plain stores are faster: the next step reads the field from the cache.
+template <class B>
+struct Streamed : B {
+  using T = typename B::T;
+  __device__ static inline void st(T *p, uint32_t i, float v, float w) { __builtin_nontemporal_store(B::enc(v, w), p + i); }
+};
+constexpr size_t kLastLevelCacheBytes = size_t(256) << 20;
 
 inline size_t store_bytes(int s) { return s == VSX_STORE_F32 ? 4 : 2; }
 

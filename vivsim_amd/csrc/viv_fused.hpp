@@ -387,6 +387,11 @@ inline bool walls_allow_push(const VsxLbm &p) {
 }
 
 // ------------------------------------------------------------------ launches
+template <class B>
+struct IsStreamed : std::false_type {};
+template <class B>
+struct IsStreamed<Streamed<B>> : std::true_type {};
+
 Tiles make_tiles(const StepCtx &c);
 
 // Bulk blocks take two tiles (PAIR) for the 16-bit storages on grids of at
@@ -412,9 +417,11 @@ int launch_fused_p(const StepCtx &c, const VsxIB2D &ib, const void *src, void *d
   if (pipe)
     hipLaunchKernelGGL((k_viv_fused<SRC, DST, COLL, PAIR, MODE, true>), dim3(grid), dim3(kTileCells), lds, s, c, ib,
                        static_cast<const typename SRC::T *>(src), static_cast<typename DST::T *>(dst), step, T);
-  else
+  else if constexpr (!IsStreamed<DST>::value)
     hipLaunchKernelGGL((k_viv_fused<SRC, DST, COLL, PAIR, MODE, false>), dim3(1 + grid), dim3(kTileCells), lds, s, c,
                        ib, static_cast<const typename SRC::T *>(src), static_cast<typename DST::T *>(dst), step, T);
+  else
+    return fail(VSX_ERR_VALUE, "non-temporal field stores are built for the pipelined form only");
   timer_end(s);
   VSX_CHECK_LAUNCH();
   return VSX_OK;
@@ -423,6 +430,15 @@ int launch_fused_p(const StepCtx &c, const VsxIB2D &ib, const void *src, void *d
 template <class SRC, class DST, int COLL, int MODE>
 int launch_fused_c(const StepCtx &c, const VsxIB2D &ib, const void *src, void *dst, int step, hipStream_t s) {
   const Tiles T = make_tiles(c);
+  if constexpr (MODE == kModeFull) {
+    // full steps of the pipelined form whose work buffer alone exceeds the
+    // last-level cache: the field stores are non-temporal (such a 16-bit field
+    // has more than kPairMinTiles tiles, so its bulk blocks take tile pairs).
+    // Buffers that exceed the cache only together (2048^2 FP32, 2 x 151 MB)
+    // still gain from plain stores.
+    if (ib.fused == 2 && (size_t)c.n * 9 * sizeof(typename DST::T) > kLastLevelCacheBytes)
+      return launch_fused_p<SRC, Streamed<DST>, COLL, sizeof(typename SRC::T) == 2, MODE>(c, ib, src, dst, step, s, T);
+  }
   if constexpr (MODE == kModeFull && sizeof(typename SRC::T) == 2) {
     if (T.ntx * T.nty >= kPairMinTiles) return launch_fused_p<SRC, DST, COLL, true, MODE>(c, ib, src, dst, step, s, T);
   }
