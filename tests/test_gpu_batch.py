@@ -96,6 +96,28 @@ def test_batch_equals_single_case_runs(sim):
         assert_close(b.view(got, k), to_np(ref).astype(np.float64), 1e-5, f"case {k}")
 
 
+def test_batch_streamed_stores_equal_single_case_runs(sim):
+    """A batch whose work buffer exceeds 256 MiB (the last-level cache) writes
+    its full steps with non-temporal stores: two 2752 x 1536 cases (304 MB in
+    FP32), each equal to its own VIV2D run (whose 152 MB buffer takes plain
+    stores)."""
+    steps = 4
+    pcs = [sweep_case(sim.VIVCase, 2752, 1536, 2.0, 4.0, 150.0, D=40.0, n_markers=168, cx=688.0, cy=768.0),
+           sweep_case(sim.VIVCase, 2752, 1536, 3.0, 6.0, 300.0, D=40.0, n_markers=168, cx=700.0, cy=760.0)]
+    b = sim.VIVBatch(pcs)
+    assert b.total * 4 > 256 * 2 ** 20
+    b.set_bodies(v=(0.0, 1e-3))
+    f0 = b.initial_f()
+    say("batch streamed stores 2 x 2752x1536")
+    got = b.run(f0, steps)
+    b.check()
+    for k, c in enumerate(pcs):
+        one = sim.VIV2D(c)
+        one.set_body(v=(0.0, 1e-3))
+        ref, _, _ = one.run(b.view(f0, k).clone(), steps)
+        assert_close(b.view(got, k), to_np(ref).astype(np.float64), 1e-5, f"streamed case {k}")
+
+
 def test_batch_graph_replays(sim):
     """Replays of a captured 8-step graph continue the clock, body slots and
     histories: 3 replays == one 24-step run."""

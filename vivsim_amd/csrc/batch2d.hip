@@ -168,6 +168,16 @@ void launch_batch_c(const VsxVivBatch &b, const void *src, void *dst, int step, 
   }
 }
 
+template <class SRC, class DST, int PHASE>
+void launch_batch_coll(const VsxVivBatch &b, const void *src, void *dst, int step, hipStream_t s) {
+  switch (b.collision) {
+    case VSX_COLL_BGK: return launch_batch_c<SRC, DST, VSX_COLL_BGK, PHASE>(b, src, dst, step, s);
+    case VSX_COLL_MRT: return launch_batch_c<SRC, DST, VSX_COLL_MRT, PHASE>(b, src, dst, step, s);
+    case VSX_COLL_KBC: return launch_batch_c<SRC, DST, VSX_COLL_KBC, PHASE>(b, src, dst, step, s);
+    default: return launch_batch_c<SRC, DST, VSX_COLL_REG, PHASE>(b, src, dst, step, s);
+  }
+}
+
 template <class ST, int PHASE>
 void launch_batch_p(const VsxVivBatch &b, const void *src, void *dst, int step, hipStream_t s) {
   using F32 = Store<VSX_STORE_F32>;
@@ -176,12 +186,12 @@ void launch_batch_p(const VsxVivBatch &b, const void *src, void *dst, int step, 
   if constexpr (PHASE == VSX_PHASE_STREAM) {
     launch_batch_c<SRC, DST, VSX_COLL_BGK, PHASE>(b, src, dst, step, s);
   } else {
-    switch (b.collision) {
-      case VSX_COLL_BGK: return launch_batch_c<SRC, DST, VSX_COLL_BGK, PHASE>(b, src, dst, step, s);
-      case VSX_COLL_MRT: return launch_batch_c<SRC, DST, VSX_COLL_MRT, PHASE>(b, src, dst, step, s);
-      case VSX_COLL_KBC: return launch_batch_c<SRC, DST, VSX_COLL_KBC, PHASE>(b, src, dst, step, s);
-      default: return launch_batch_c<SRC, DST, VSX_COLL_REG, PHASE>(b, src, dst, step, s);
+    // full steps whose work buffer exceeds the last-level cache: non-temporal field stores
+    if constexpr (PHASE == VSX_PHASE_FULL) {
+      if ((size_t)b.total_pops * sizeof(typename ST::T) > kLastLevelCacheBytes)
+        return launch_batch_coll<SRC, Streamed<DST>, PHASE>(b, src, dst, step, s);
     }
+    launch_batch_coll<SRC, DST, PHASE>(b, src, dst, step, s);
   }
 }
 
