@@ -179,6 +179,19 @@ def test_fused3d_half(sim, storage, tol):
     assert_close(got, ref, tol, storage)
 
 
+def test_fused3d_streamed_stores(sim):
+    """A full step whose work buffer exceeds 256 MiB (the last-level cache)
+    writes the field with non-temporal stores: 160 x 160 x 140 in FP32
+    (272 MB), MRT, the six-wall box, 2 steps against the oracle."""
+    sh, n, om = (160, 160, 140), 2, 1.8
+    f0 = field(sh)
+    walls = WALLSETS3["box"]
+    say("fused d3q19 streamed stores 160x160x140")
+    got = sim.LBM(sh, om, collision="mrt", walls=walls).run(dev(f0), n)
+    ref = run_oracle(f0, n, om, walls, "mrt", None, None)
+    assert_close(got, ref, 3e-5, "streamed stores 3d")
+
+
 # ------------------------------------------------------------------ sphere IB-LBM (3D)
 
 def small_sphere(cls, **kw):

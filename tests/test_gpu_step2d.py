@@ -126,6 +126,19 @@ def test_cavity_1000_steps(sim):
     say(f"cavity rel err f {e:.2e}")
 
 
+def test_streamed_stores(sim):
+    """A full step whose work buffer exceeds 256 MiB (the last-level cache)
+    writes the field with non-temporal stores: 2760 x 2740 in FP32 (272 MB),
+    KBC, the mixed wall set, 2 steps against the oracle."""
+    nx, ny, n, om = 2760, 2740, 2, 1.7
+    f0 = rand_state(nx, ny)
+    walls = WALLSETS["mixed"]
+    say("fused d2q9 streamed stores 2760x2740")
+    got = sim.LBM((nx, ny), om, collision="kbc", walls=walls).run(dev(f0), n)
+    ref = run_oracle(f0, n, om, walls, "kbc", None, None)
+    assert_close(got, ref, 2e-5, "streamed stores 2d")
+
+
 @pytest.mark.parametrize("storage,tol", [("f16s", 2e-3), ("bf16s", 2e-2), ("e3m12s", 1e-3)])
 def test_half_storage(sim, storage, tol):
     nx, ny, n, om = 40, 32, 30, 1.7

@@ -114,6 +114,9 @@ int phase_dispatch(const StepCtx &c, const void *src, bool src_f32, void *dst, b
       return src_f32 ? launch_phase<L, VSX_PHASE_COLLIDE, F32, ST>(c, src, dst, s)
                      : launch_phase<L, VSX_PHASE_COLLIDE, ST, ST>(c, src, dst, s);
     case VSX_PHASE_FULL:
+      // a work buffer larger than the last-level cache: non-temporal field stores
+      if ((size_t)c.n * L::Q * sizeof(typename ST::T) > kLastLevelCacheBytes)
+        return launch_phase<L, VSX_PHASE_FULL, ST, Streamed<ST>>(c, src, dst, s);
       return launch_phase<L, VSX_PHASE_FULL, ST, ST>(c, src, dst, s);
     case VSX_PHASE_STREAM:
       return dst_f32 ? launch_phase<L, VSX_PHASE_STREAM, ST, F32>(c, src, dst, s)
