@@ -114,9 +114,13 @@ int phase_dispatch(const StepCtx &c, const void *src, bool src_f32, void *dst, b
       return src_f32 ? launch_phase<L, VSX_PHASE_COLLIDE, F32, ST>(c, src, dst, s)
                      : launch_phase<L, VSX_PHASE_COLLIDE, ST, ST>(c, src, dst, s);
     case VSX_PHASE_FULL:
-      // a work buffer larger than the last-level cache: non-temporal field stores
-      if ((size_t)c.n * L::Q * sizeof(typename ST::T) > kLastLevelCacheBytes)
-        return launch_phase<L, VSX_PHASE_FULL, ST, Streamed<ST>>(c, src, dst, s);
+      // an FP32 work buffer larger than the last-level cache: non-temporal
+      // field stores (FP32 only, which keeps the kernel count of the generic
+      // step down; a 16-bit field needs twice the cells to reach that size)
+      if constexpr (std::is_same_v<ST, Store<VSX_STORE_F32>>) {
+        if ((size_t)c.n * L::Q * sizeof(float) > kLastLevelCacheBytes)
+          return launch_phase<L, VSX_PHASE_FULL, ST, Streamed<ST>>(c, src, dst, s);
+      }
       return launch_phase<L, VSX_PHASE_FULL, ST, ST>(c, src, dst, s);
     case VSX_PHASE_STREAM:
       return dst_f32 ? launch_phase<L, VSX_PHASE_STREAM, ST, F32>(c, src, dst, s)
